@@ -1,0 +1,232 @@
+"""Pins for the CPU oracle (no GPU).
+
+Every oracle function is checked against something it does not share code with:
+  * an independent Jordan-Wigner brute force (oracle/jw.py) element by element,
+  * the energies the reference's own tests hold (test_routines.py:37-54),
+  * the closed-form RDM tests of test_routines.py:494-800,
+  * combinatorial identities (colex rank/unrank, C(n,k) counts).
+A planted mistake (dropped term, wrong sign, transposed operand) fails one of them.
+"""
+import itertools
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+from oracle import fcidump, jw
+from tests.conftest import datafile
+from pyci_amd.synthetic import seeded_integrals
+
+
+def test_colex_rank_unrank_roundtrip(oracle):
+    for n, k in [(8, 3), (12, 6), (14, 7), (48, 2)]:
+        total = oracle.binomial(n, k)
+        ranks = np.unique(np.linspace(0, total - 1, 200).astype(np.int64))
+        prev = -1
+        for r in ranks:
+            d = oracle.unrank_colex(n, k, int(r))
+            assert bin(d).count("1") == k and d < (1 << n)
+            assert oracle.rank_colex(n, k, d) == r
+        # colex order = numeric order of the bit strings
+        ds = [oracle.unrank_colex(n, k, int(r)) for r in ranks]
+        assert all(a < b for a, b in zip(ds, ds[1:]))
+
+
+def test_all_dets_counts_and_order(oracle):
+    d = oracle.all_dets("fullci", 6, 3, 2)
+    assert d.shape == (20 * 15, 2)
+    # alpha colex rank major, beta minor (twospinwfn.cpp:181-221)
+    assert all(oracle.rank_colex(6, 3, int(d[i, 0])) == i // 15 for i in range(0, 300, 7))
+    assert all(oracle.rank_colex(6, 2, int(d[i, 1])) == i % 15 for i in range(0, 300, 7))
+
+
+def test_phases_match_operator_algebra(oracle):
+    n = 7
+    for det in [0b1011001, 0b0110110, 0b1110001]:
+        occ = [p for p in range(n) if det >> p & 1]
+        vir = [p for p in range(n) if not det >> p & 1]
+        for i in occ:
+            for a in vir:
+                s, st, ok = jw._apply([(a, True), (i, False)], np.array([det]))
+                assert ok[0] and oracle.phase_single(det, i, a) == s[0]
+        for i, k in itertools.combinations(occ, 2):
+            for a, l in itertools.combinations(vir, 2):
+                s, st, ok = jw._apply([(a, True), (l, True), (k, False), (i, False)], np.array([det]))
+                assert ok[0] and oracle.phase_double(det, i, k, a, l) == s[0]
+
+
+@pytest.mark.parametrize("nu,nd", [(1, 1), (2, 1), (2, 2), (3, 2)])
+def test_fullci_csr_equals_brute_force(oracle, nu, nd):
+    _, one, two = seeded_integrals(4, 6, 7)
+    dets = oracle.all_dets("fullci", 4, nu, nd)
+    h, g = jw.spin_orbital_integrals(one, two)
+    H = jw.hamiltonian(h, g, jw.fullci_basis(dets, 4))
+    csr = oracle.csr_build("fullci", 4, nu, nd, dets, one, two)
+    npt.assert_allclose(csr.dense(), H, rtol=0, atol=1e-13)
+    full = oracle.csr_build("fullci", 4, nu, nd, dets, one, two, symmetric=False)
+    npt.assert_allclose(full.dense(), H, rtol=0, atol=1e-13)
+    # lower storage: only col <= row, rows sorted, diagonal last
+    for i in range(csr.shape[0]):
+        row = csr.indices[csr.indptr[i]:csr.indptr[i + 1]]
+        assert np.all(np.diff(row) > 0) and row[-1] == i
+
+
+@pytest.mark.parametrize("no", [1, 2, 3])
+def test_doci_csr_equals_brute_force(oracle, no):
+    _, one, two = seeded_integrals(4, 6, 8)
+    dets = oracle.all_dets("doci", 4, no, no)
+    h, g = jw.spin_orbital_integrals(one, two)
+    H = jw.hamiltonian(h, g, jw.doci_basis(dets, 4))
+    npt.assert_allclose(oracle.csr_build("doci", 4, no, no, dets, one, two).dense(), H, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("ne", [2, 3, 4])
+def test_genci_csr_equals_brute_force(oracle, ne):
+    _, one, two = seeded_integrals(4, 6, 9)
+    h, g = jw.spin_orbital_integrals(one, two)
+    dets = oracle.all_dets("genci", 8, ne, 0)
+    H = jw.hamiltonian(h, g, dets.astype(np.int64))
+    npt.assert_allclose(oracle.csr_build("genci", 8, ne, 0, dets, h, g).dense(), H, rtol=0, atol=1e-13)
+
+
+def test_rectangular_and_subset(oracle):
+    """A subset wave function (HF + singles + doubles) and a rectangular nrow x ncol
+    block: entries are the brute-force H restricted to those rows/cols."""
+    _, one, two = seeded_integrals(5, 6, 10)
+    hf = np.array([0b00011, 0b00011], dtype=np.uint64)
+    dets = np.zeros((0, 2), dtype=np.uint64)
+    for e in (0, 1, 2):
+        dets = oracle.add_excited_dets("fullci", 5, 2, 2, dets, hf, e)
+    h, g = jw.spin_orbital_integrals(one, two)
+    H = jw.hamiltonian(h, g, jw.fullci_basis(dets, 5))
+    nrow = len(dets) - 10
+    rect = oracle.csr_build("fullci", 5, 2, 2, dets, one, two, nrow=nrow, symmetric=False)
+    npt.assert_allclose(rect.dense(), H[:nrow], rtol=0, atol=1e-13)
+
+
+def _energy(oracle, fn, kind, occ):
+    ec, one, two = fcidump.read(datafile(fn + ".fcidump"))
+    n = one.shape[0]
+    dets = oracle.all_dets(kind, n, *occ)
+    csr = oracle.csr_build(kind, n, *occ, dets, one, two)
+    w, v = oracle.lowest_eigs(csr, 1)
+    return w[0] + ec, dets, v[0], (ec, one, two)
+
+
+@pytest.mark.parametrize("fn,kind,occ,energy", [
+    ("be_ccpvdz", "doci", (2, 2), -14.600556994),
+    ("be_ccpvdz", "fullci", (2, 2), -14.617409507),
+    ("li2_ccpvdz", "doci", (3, 3), -14.878455349),
+])
+def test_reference_energies(oracle, fn, kind, occ, energy):
+    """Known answers held by the reference (test_routines.py:37-54, atol 1e-9)."""
+    e, *_ = _energy(oracle, fn, kind, occ)
+    npt.assert_allclose(e, energy, rtol=0, atol=1e-9)
+
+
+def test_h4_doci_spectrum(oracle):
+    """H4/STO-3G DOCI spectrum (all six roots) from an independent dense brute force."""
+    ec, one, two = fcidump.read(datafile("h4_sto3g.fcidump"))
+    dets = oracle.all_dets("doci", 4, 2, 2)
+    h, g = jw.spin_orbital_integrals(one, two)
+    H = jw.hamiltonian(h, g, jw.doci_basis(dets, 4))
+    w_ref = np.linalg.eigvalsh(H) + ec
+    w = np.linalg.eigvalsh(oracle.csr_build("doci", 4, 2, 2, dets, one, two).dense()) + ec
+    npt.assert_allclose(w, w_ref, atol=1e-12)
+    npt.assert_allclose(w, [-2.13398665, -1.3971786, -0.8372566, -0.26165295, 0.10356075, 0.70975469], atol=1e-7)
+
+
+@pytest.mark.parametrize("nu,nd", [(1, 1), (2, 1), (2, 2)])
+def test_rdms_equal_brute_force(oracle, nu, nd):
+    n = 3
+    dets = oracle.all_dets("fullci", n, nu, nd)
+    c = np.random.default_rng(1).normal(size=len(dets))
+    c /= np.linalg.norm(c)
+    R1, R2 = jw.rdms(c, jw.fullci_basis(dets, n), 2 * n)
+    r1, r2 = oracle.rdms("fullci", n, dets, c)
+    npt.assert_allclose(r1[0], R1[:n, :n], atol=1e-14)
+    npt.assert_allclose(r1[1], R1[n:, n:], atol=1e-14)
+    npt.assert_allclose(r2[0], R2[:n, :n, :n, :n], atol=1e-14)
+    npt.assert_allclose(r2[1], R2[n:, n:, n:, n:], atol=1e-14)
+    npt.assert_allclose(r2[2], R2[:n, n:, :n, n:], atol=1e-14)
+    g = (dets[:, 0].astype(np.int64) | (dets[:, 1].astype(np.int64) << n)).astype(np.uint64)
+    g1, g2 = oracle.rdms("genci", 2 * n, g, c)
+    npt.assert_allclose(g1, R1, atol=1e-14)
+    npt.assert_allclose(g2, R2, atol=1e-14)
+
+
+def test_doci_rdms_brute_force(oracle):
+    dets = oracle.all_dets("doci", 4, 2, 2)
+    c = np.random.default_rng(2).normal(size=len(dets))
+    c /= np.linalg.norm(c)
+    d0, d2 = oracle.rdms("doci", 4, dets, c)
+    R1, R2 = jw.rdms(c, jw.doci_basis(dets, 4), 8)
+    # d0[p,q] = <P+_p P_q> (pair hop), d2[p,q] = <n_pa n_qa> for p != q
+    npt.assert_allclose(d0, [[R2[p, p + 4, q, q + 4] for q in range(4)] for p in range(4)], atol=1e-14)
+    npt.assert_allclose(d2, [[0.0 if p == q else R2[p, q, p, q] for q in range(4)] for p in range(4)], atol=1e-14)
+
+
+def test_rdm_closed_form_two_particles(oracle):
+    """test_routines.py:494-532: FullCI(2,1,1), c = sqrt(1..4)/norm."""
+    dets = oracle.all_dets("fullci", 2, 1, 1)
+    c = np.sqrt(np.arange(1.0, 5.0))
+    c /= np.linalg.norm(c)
+    _, d = oracle.rdms("fullci", 2, dets, c)
+    ab = d[2]
+    npt.assert_allclose([ab[0, 0, 0, 0], ab[0, 1, 0, 1], ab[1, 0, 1, 0], ab[1, 1, 1, 1]], c ** 2, atol=1e-12)
+    npt.assert_allclose([ab[0, 0, 0, 1], ab[0, 0, 1, 0], ab[0, 0, 1, 1]], [c[0] * c[1], c[0] * c[2], c[0] * c[3]], atol=1e-12)
+    npt.assert_allclose([ab[1, 1, 0, 0], ab[1, 1, 0, 1], ab[1, 1, 1, 0]], [c[3] * c[0], c[3] * c[1], c[3] * c[2]], atol=1e-12)
+    assert np.abs(d[0]).max() < 1e-12 and np.abs(d[1]).max() < 1e-12
+
+
+def test_rdm_closed_form_two_up_one_dn(oracle):
+    """test_routines.py:695-800: FullCI(3,2,1), c = sqrt(1..9)/norm."""
+    dets = oracle.all_dets("fullci", 3, 2, 1)
+    c = np.sqrt(np.arange(1.0, 10.0))
+    c /= np.linalg.norm(c)
+    _, d = oracle.rdms("fullci", 3, dets, c)
+    a = d[0]
+    npt.assert_allclose(a[0, 1, 0, 1], c[0] ** 2 + c[1] ** 2 + c[2] ** 2, atol=1e-12)
+    npt.assert_allclose(a[1, 2, 1, 2], c[6] ** 2 + c[7] ** 2 + c[8] ** 2, atol=1e-12)
+    npt.assert_allclose(a[0, 1, 1, 2], c[0] * c[6] + c[1] * c[7] + c[2] * c[8], atol=1e-12)
+    ab = d[2]
+    npt.assert_allclose(ab[0, 0, 2, 0], -c[6] * c[0], atol=1e-12)
+    npt.assert_allclose(ab[1, 2, 1, 0], c[0] * c[2] + c[6] * c[8], atol=1e-12)
+    npt.assert_allclose(ab[2, 2, 2, 2], c[8] ** 2 + c[5] ** 2, atol=1e-12)
+
+
+def test_be_rdm_energy(oracle):
+    """Energy recomputed from spin-orbital RDMs with the reference-held
+    be_ccpvdz_spinres.npz integrals (test_routines.py:115-133)."""
+    e, dets, c, (ec, one, two) = _energy(oracle, "be_ccpvdz", "fullci", (2, 2))
+    r1, r2 = oracle.rdms("fullci", one.shape[0], dets, c)
+    from pyci_amd.utility import spinize_rdms
+
+    R1, R2 = spinize_rdms(r1, r2)
+    with np.load(datafile("be_ccpvdz_spinres.npz")) as f:
+        so1, so2 = f["one_mo"], f["two_mo"]
+    e2 = ec + np.einsum("ij,ij", so1, R1) + 0.25 * np.einsum("ijkl,ijkl", so2, R2)
+    npt.assert_allclose(e2, e, atol=1e-9)
+    npt.assert_allclose(np.trace(R1), 4.0, atol=1e-12)
+
+
+def test_hci_matches_brute_force_criterion(oracle):
+    """add_hci adds exactly {J not in wfn : |H_JI| > eps/|c_I| for some I in wfn}
+    (hci.cpp:49-182); the brute-force H supplies |H_JI| independently."""
+    _, one, two = seeded_integrals(5, 6, 12)
+    alld = oracle.all_dets("fullci", 5, 2, 2)
+    h, g = jw.spin_orbital_integrals(one, two)
+    H = jw.hamiltonian(h, g, jw.fullci_basis(alld, 5))
+    hf = np.array([[0b00011, 0b00011]], dtype=np.uint64)
+    dets = oracle.add_excited_dets("fullci", 5, 2, 2, hf, hf[0], 1)
+    idx = [int(np.where((alld == d).all(1))[0][0]) for d in dets]
+    c = np.random.default_rng(3).normal(size=len(dets))
+    for eps in (0.0, 1e-3, 2e-2):
+        new = oracle.add_hci("fullci", 5, 2, 2, dets, one, two, c, eps)
+        got = {tuple(x) for x in new}
+        want = set()
+        for col, i in enumerate(idx):
+            for j in range(len(alld)):
+                if j not in idx and abs(H[j, i]) > eps / abs(c[col]):
+                    want.add(tuple(alld[j]))
+        assert got == want and len(new) == len(got)
