@@ -1,0 +1,162 @@
+/*
+ * pyci_engine.h -- C ABI of the MI355X (gfx950) configuration-interaction engine.
+ *
+ * This is the drop-in boundary beneath the pybind11 module `pyci_amd._pyci`
+ * (the reference's `pyci._pyci`, pyci/src/binding.cpp:29-1507). Plain pointers
+ * and sizes only; no torch or pybind11 types. Every call returns 0 on success or
+ * one of the PYCI_E* codes below; pyci_last_error() then holds a message for the
+ * calling thread. Host pointers are never retained after a call returns; device
+ * memory belongs to the handle that allocated it and is freed by its *_destroy.
+ * Calls on one context are serialised on that context's HIP stream and are not
+ * re-entrant; different contexts may be used from different threads.
+ *
+ * Determinant layout (as the reference's `dets` vectors, one 64-bit word per
+ * spin string): DOCI and GenCI dets are one uint64 each; FullCI dets are two
+ * uint64 words (alpha string, beta string). The GPU kernels support nbasis <= 64
+ * (DOCI/FullCI spatial orbitals) and nbasis <= 64 spin-orbitals (GenCI).
+ */
+#ifndef PYCI_ENGINE_H
+#define PYCI_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; the pybind11 layer re-raises them as the reference does
+ * (std::invalid_argument/domain_error -> ValueError, pybind11::type_error ->
+ * TypeError, std::runtime_error -> RuntimeError; sparseop.cpp:116-136). */
+#define PYCI_OK 0
+#define PYCI_EINVAL 1      /* ValueError */
+#define PYCI_ETYPE 2       /* TypeError */
+#define PYCI_ENOCONV 3     /* RuntimeError("did not converge") */
+#define PYCI_EHIP 4        /* RuntimeError: HIP runtime failure or no GPU */
+#define PYCI_ENOMEM 5      /* MemoryError */
+#define PYCI_EUNSUPPORTED 6 /* ValueError: shape outside the GPU kernels' limits */
+
+/* Wave-function kinds (DOCIWfn, FullCIWfn, GenCIWfn; pyci.h:306-617). */
+#define PYCI_WFN_DOCI 0
+#define PYCI_WFN_FULLCI 1
+#define PYCI_WFN_GENCI 2
+
+/* Eigensolver precision modes for pyci_op_solve. */
+#define PYCI_PREC_F64 0   /* FP64 sigma throughout */
+#define PYCI_PREC_MIXED 1 /* FP32-value sigma, then FP64 residual-correction pass */
+
+typedef struct pyci_ctx pyci_ctx; /* one device + one HIP stream */
+typedef struct pyci_ham pyci_ham; /* integrals resident in HBM */
+typedef struct pyci_wfn pyci_wfn; /* determinant set + device index */
+typedef struct pyci_op pyci_op;   /* sparse CI operator (CSR) in HBM */
+
+/* Solver statistics filled by pyci_op_solve (may be NULL). */
+typedef struct pyci_solve_stats {
+    int64_t iterations;     /* Davidson iterations (all phases) */
+    int64_t sigma_calls;    /* H*X products, counted per vector */
+    int64_t sigma_calls_f32;/* of which FP32-value sigma */
+    double residual[64];    /* final residual 2-norm per root (first 64 roots) */
+} pyci_solve_stats;
+
+/* Message of the last failing call on this thread (never NULL). */
+const char *pyci_last_error(void);
+
+/* Number of visible GPUs (0 on a host without GPUs; not an error). */
+int pyci_device_count(int *count);
+
+/* Context: binds a device and creates its stream. No counterpart in the
+ * reference (its SparseOp runs on the host thread that calls it). */
+int pyci_ctx_create(int device, pyci_ctx **ctx);
+void pyci_ctx_destroy(pyci_ctx *ctx);
+int pyci_ctx_synchronize(pyci_ctx *ctx);
+/* HIP events recorded on the context stream (for measurement). */
+int pyci_ctx_timer_start(pyci_ctx *ctx);
+int pyci_ctx_timer_stop(pyci_ctx *ctx, float *ms);
+
+/* Integrals. Replaces SQuantOp(ecore, one_mo, two_mo) (squantop.cpp:163-183):
+ * one_mo (n,n) and physicist two_mo (n,n,n,n) <pq|rs>, C order, float64. The
+ * arrays are copied into HBM. */
+int pyci_ham_create(pyci_ctx *ctx, int64_t nbasis, double ecore, const double *one_mo, const double *two_mo,
+                    pyci_ham **ham);
+void pyci_ham_destroy(pyci_ham *ham);
+
+/* Determinant set. Replaces the `dets` vector and hash `dict` of OneSpinWfn /
+ * TwoSpinWfn (onespinwfn.cpp:57-63,123-143; twospinwfn.cpp:57-63,129-147).
+ * `dets` must hold ndet distinct determinants with the stated occupations; the
+ * insertion index of each is its position. A set equal to the full space in
+ * colex order is indexed by perfect colex rank, any other by a GPU hash table. */
+int pyci_wfn_create(pyci_ctx *ctx, int kind, int64_t nbasis, int64_t nocc_up, int64_t nocc_dn, int64_t ndet,
+                    const uint64_t *dets, pyci_wfn **wfn);
+/* Appends n determinants that are not yet in the set (add_det, onespinwfn.cpp:136-143). */
+int pyci_wfn_append(pyci_wfn *wfn, int64_t n, const uint64_t *dets);
+int pyci_wfn_ndet(const pyci_wfn *wfn, int64_t *ndet);
+/* Copies dets[start:end] to host memory (to_det_array, onespinwfn.cpp:106-110). */
+int pyci_wfn_get_dets(const pyci_wfn *wfn, int64_t start, int64_t end, uint64_t *out);
+/* Batched index_det (onespinwfn.cpp:123-126): -1 where absent. */
+int pyci_wfn_index(const pyci_wfn *wfn, int64_t n, const uint64_t *dets, int64_t *index);
+void pyci_wfn_destroy(pyci_wfn *wfn);
+
+/* Fills the full determinant space in colex order into host memory `out`
+ * (add_all_dets: onespinwfn.cpp:173-217, twospinwfn.cpp:181-245). Words per
+ * spin string nword = ceil(nbasis/64) (any nbasis); FullCI writes
+ * 2*nword words per det (alpha then beta), others nword. */
+int pyci_fill_all_dets(pyci_ctx *ctx, int kind, int64_t nbasis, int64_t nocc_up, int64_t nocc_dn, uint64_t *out);
+
+/* Sparse CI operator. Replaces SparseOp(ham, wfn, nrow, ncol, symmetric)
+ * (sparseop.cpp:57-63 -> update :186-201 -> add_row :220-502 -> sort_row :214-218).
+ * nrow/ncol < 0 mean ndet. The operator holds every structural element of rows
+ * 0..nrow-1 with column < ncol, rows sorted by column; with symmetric != 0 the
+ * reference-format export is the lower triangle plus diagonal. */
+int pyci_op_create(pyci_ham *ham, pyci_wfn *wfn, int64_t nrow, int64_t ncol, int symmetric, pyci_op **op);
+/* update(ham, wfn) (sparseop.cpp:175-178): extends the operator to the wave
+ * function's current ndet rows and columns. */
+int pyci_op_update(pyci_op *op, pyci_ham *ham, pyci_wfn *wfn);
+/* shape, size (reference nnz), stored nnz, symmetric flag, ecore. */
+int pyci_op_info(const pyci_op *op, int64_t *nrow, int64_t *ncol, int64_t *size, int64_t *nnz_stored,
+                 int *symmetric, double *ecore);
+/* data()/indices()/indptr() in the reference format (sparseop.cpp:504-514):
+ * size float64 values, size int64 columns, nrow+1 int64 row pointers. */
+int pyci_op_export(pyci_op *op, double *data, int64_t *indices, int64_t *indptr);
+/* get_element(i, j) (sparseop.cpp:89-94); 0 for j > i in symmetric mode. */
+int pyci_op_get_element(pyci_op *op, int64_t i, int64_t j, double *value);
+/* Diagonal H_ii for every row (host output, nrow values, ecore not added). */
+int pyci_op_diagonal(pyci_op *op, double *diag);
+/* y = A x for nvec vectors (perform_op / perform_op_symm, sparseop.cpp:96-112).
+ * x is nvec x ncol and y nvec x nrow, row-major (one vector per row), host memory. */
+int pyci_op_matvec(pyci_op *op, int64_t nvec, const double *x, double *y);
+/* Same with device pointers (x, y in HBM, X row-major ncol x nvec, i.e. the nvec
+ * values of one determinant are contiguous); runs on the context stream. */
+int pyci_op_matvec_device(pyci_op *op, int64_t nvec, const double *x_dev, double *y_dev);
+/* Timed sigma for measurement: nrep launches of the SpMM kernel on device-resident
+ * random X (nvec columns); returns the average kernel time from HIP events and the
+ * algorithmic bytes of one launch. fp32 != 0 times the FP32-value kernel. */
+int pyci_op_bench_sigma(pyci_op *op, int64_t nvec, int64_t nrep, int fp32, float *avg_ms, double *bytes);
+/* solve(n, c0, ncv, maxiter, tol) (sparseop.cpp:114-146,161-173): lowest n
+ * eigenpairs by block Davidson on the GPU. evals (n, ecore added) and evecs
+ * (n, nrow) are returned in the reference order: descending eigenvalue when
+ * n > 1. c0 (nrow) or NULL. ncv/maxiter < 0 select the reference defaults
+ * ncv = min(nrow, max(2n+1, 20)) and maxiter = n*nrow*10. */
+int pyci_op_solve(pyci_op *op, int64_t n, const double *c0, int64_t ncv, int64_t maxiter, double tol, int precision,
+                  double *evals, double *evecs, pyci_solve_stats *stats);
+void pyci_op_destroy(pyci_op *op);
+
+/* Heat-bath selection. Replaces add_hci (hci.cpp:252-279, per-det screening
+ * :49-238): appends to `wfn` every determinant connected to some det i by an
+ * excitation with |H_ji| > eps/|c_i| that is not already present. New
+ * determinants are appended in first-occurrence order of the serial
+ * enumeration (source det ascending, reference loop order). coeffs: ndet
+ * float64 in host memory. */
+int pyci_add_hci(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs, double eps, int64_t *nadded);
+
+/* compute_rdms (rdm.cpp:1020-1056). coeffs: ndet float64, host memory.
+ *   DOCI:   rdm1 = d0 (n,n),   rdm2 = d2 (n,n)          (rdm.cpp:20-65)
+ *   FullCI: rdm1 = (2,n,n),    rdm2 = (3,n,n,n,n)      (rdm.cpp:269-530)
+ *   GenCI:  rdm1 = (n,n),      rdm2 = (n,n,n,n)        (mathematically exact
+ *           <a+_p a_q>, <a+_p a+_q a_s a_r>; the reference routine at
+ *           rdm.cpp:532-632 has index errors, so it is not reproduced). */
+int pyci_compute_rdms(pyci_wfn *wfn, const double *coeffs, double *rdm1, double *rdm2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYCI_ENGINE_H */

@@ -1,0 +1,374 @@
+// pybind11 module `pyci_amd._pyci`: the reference's `pyci._pyci` surface
+// (pyci/src/binding.cpp:29-1507) for the integrals, wave functions, sparse
+// operator, heat-bath selection and RDMs, backed by the GPU engine.
+#include <cstring>
+
+#include "host.hpp"
+
+namespace ph {
+static i64 g_threads = 1;
+
+template <class W>
+static void copy_from(W &dst, const Wfn &src) {
+    dst.dets = src.dets;
+    dst.ndet = src.ndet;
+    dst.index_valid = false;
+    dst.full_colex = src.full_colex;
+}
+
+template <class W>
+static std::unique_ptr<W> from_dets(i64 nb, i64 nu, i64 nd, Array<u64> a) {
+    auto w = std::make_unique<W>(nb, nu, nd);
+    i64 n = a.ndim() ? a.shape(0) : 0;
+    if ((i64)a.size() != n * w->nwdet) throw py::value_error("determinant array has the wrong shape");
+    w->dets.assign(a.data(), a.data() + n * w->nwdet);
+    w->ndet = n;
+    w->index_valid = false;
+    return w;
+}
+
+template <class W>
+static std::unique_ptr<W> from_occs(i64 nb, i64 nu, i64 nd, Array<i64> a) {
+    auto w = std::make_unique<W>(nb, nu, nd);
+    const bool two = (w->kind == PYCI_WFN_FULLCI);
+    const i64 per = two ? 2 * nu : nu;
+    i64 n = a.ndim() ? a.shape(0) : 0;
+    if ((i64)a.size() != n * per) throw py::value_error("occupation array has the wrong shape");
+    w->dets.assign((size_t)(n * w->nwdet), 0ULL);
+    const i64 *p = a.data();
+    for (i64 i = 0; i < n; ++i) {
+        u64 *d = &w->dets[(size_t)(i * w->nwdet)];
+        fill_det(nu, p + i * per, d);
+        if (two) fill_det(nd, p + i * per + nu, d + w->nword);
+    }
+    w->ndet = n;
+    w->index_valid = false;
+    return w;
+}
+
+template <class W>
+static std::unique_ptr<W> from_file(const std::string &fn) {
+    auto w = std::make_unique<W>(1, 0, 0);
+    w->load_file(fn);
+    return w;
+}
+
+// FullCI from DOCI (fullciwfn.cpp:22-33): alpha string = beta string.
+static std::unique_ptr<FullCIWfn> fullci_from_doci(const DOCIWfn &d) {
+    auto w = std::make_unique<FullCIWfn>(d.nbasis, d.nocc_up, d.nocc_dn);
+    w->dets.resize((size_t)(d.ndet * w->nwdet));
+    for (i64 i = 0; i < d.ndet; ++i) {
+        std::memcpy(&w->dets[(size_t)(i * w->nwdet)], d.det_ptr(i), sizeof(u64) * d.nword);
+        std::memcpy(&w->dets[(size_t)(i * w->nwdet + d.nword)], d.det_ptr(i), sizeof(u64) * d.nword);
+    }
+    w->ndet = d.ndet;
+    w->index_valid = false;
+    return w;
+}
+
+// GenCI from FullCI (genciwfn.cpp:26-40): spin-orbital p (alpha) and nbasis+q
+// (beta). The reference fills the beta occupations from the alpha string
+// (genciwfn.cpp:37); this conversion uses the beta string.
+static std::unique_ptr<GenCIWfn> genci_from_fullci(const FullCIWfn &f) {
+    auto w = std::make_unique<GenCIWfn>(f.nbasis * 2, f.nocc, 0);
+    w->dets.assign((size_t)(f.ndet * w->nwdet), 0ULL);
+    std::vector<i64> occ((size_t)f.nocc + 1);
+    for (i64 i = 0; i < f.ndet; ++i) {
+        fill_occs(f.nword, f.det_ptr(i), occ.data());
+        fill_occs(f.nword, f.det_ptr(i) + f.nword, occ.data() + f.nocc_up);
+        for (i64 j = 0; j < f.nocc_dn; ++j) occ[(size_t)(f.nocc_up + j)] += f.nbasis;
+        fill_det(f.nocc, occ.data(), &w->dets[(size_t)(i * w->nwdet)]);
+    }
+    w->ndet = f.ndet;
+    w->index_valid = false;
+    return w;
+}
+
+static py::tuple det_array_range(const Wfn &w, i64 &lo, i64 &hi) {
+    if (lo == -1) {
+        lo = 0;
+        if (hi == -1) hi = w.ndet;
+    } else if (hi == -1) {
+        hi = lo;
+        lo = 0;
+    }
+    if (lo < 0 || hi > w.ndet || lo > hi) throw py::index_error("determinant range out of bounds");
+    return py::tuple();
+}
+
+static Array<u64> to_det_array(const Wfn &w, i64 lo, i64 hi) {
+    det_array_range(w, lo, hi);
+    const bool two = (w.kind == PYCI_WFN_FULLCI);
+    Array<u64> a = two ? Array<u64>({hi - lo, (i64)2, w.nword}) : Array<u64>({hi - lo, w.nword});
+    if (hi > lo) std::memcpy(a.mutable_data(), w.det_ptr(lo), sizeof(u64) * (size_t)((hi - lo) * w.nwdet));
+    return a;
+}
+
+static Array<i64> to_occ_array(const Wfn &w, i64 lo, i64 hi) {
+    det_array_range(w, lo, hi);
+    const bool two = (w.kind == PYCI_WFN_FULLCI);
+    Array<i64> a = two ? Array<i64>({hi - lo, (i64)2, w.nocc_up}) : Array<i64>({hi - lo, w.nocc_up});
+    i64 *p = a.mutable_data();
+    const i64 per = two ? 2 * w.nocc_up : w.nocc_up;
+    std::fill(p, p + (hi - lo) * per, 0);
+    for (i64 i = lo; i < hi; ++i) {
+        fill_occs(w.nword, w.det_ptr(i), p + (i - lo) * per);
+        if (two) fill_occs(w.nword, w.det_ptr(i) + w.nword, p + (i - lo) * per + w.nocc_up);
+    }
+    return a;
+}
+
+static Array<u64> getitem(const Wfn &w, i64 i) {
+    if (i < 0) i += w.ndet;
+    if (i < 0 || i >= w.ndet) throw py::index_error("determinant index out of range");
+    const bool two = (w.kind == PYCI_WFN_FULLCI);
+    Array<u64> a = two ? Array<u64>({(i64)2, w.nword}) : Array<u64>(w.nword);
+    std::memcpy(a.mutable_data(), w.det_ptr(i), sizeof(u64) * (size_t)w.nwdet);
+    return a;
+}
+
+static const u64 *det_arg(const Wfn &w, const Array<u64> &d) {
+    if ((i64)d.size() != w.nwdet) throw py::value_error("determinant has the wrong number of words");
+    return d.data();
+}
+
+static void add_dets_from(Wfn &dst, const Wfn &src) {
+    if (src.nwdet != dst.nwdet) throw py::value_error("incompatible wave functions");
+    for (i64 i = 0; i < src.ndet; ++i) dst.add_det(src.det_ptr(i));
+}
+
+template <class W>
+static i64 add_hci(SQuantOp &ham, W &wfn, Array<double> coeffs, double eps, i64) {
+    if ((i64)coeffs.size() != wfn.ndet) throw py::value_error("coeffs must have one entry per determinant");
+    if (ham.nbasis != wfn.nbasis) throw py::value_error("hamiltonian and wave function have different nbasis");
+    pyci_wfn *dw = wfn.device();
+    pyci_ham *dh = ham.device();
+    const i64 old = wfn.ndet;
+    int64_t added = 0;
+    {
+        py::gil_scoped_release nogil;
+        check(pyci_add_hci(dh, dw, coeffs.data(), eps, &added));
+    }
+    wfn.adopt_device_appends(old);
+    return added;
+}
+
+static py::tuple rdms(Wfn &wfn, Array<double> coeffs) {
+    if ((i64)coeffs.size() != wfn.ndet) throw py::value_error("coeffs must have one entry per determinant");
+    const i64 n = wfn.nbasis;
+    Array<double> r1, r2;
+    if (wfn.kind == PYCI_WFN_DOCI) {
+        r1 = Array<double>({n, n});
+        r2 = Array<double>({n, n});
+    } else if (wfn.kind == PYCI_WFN_FULLCI) {
+        r1 = Array<double>({(i64)2, n, n});
+        r2 = Array<double>({(i64)3, n, n, n, n});
+    } else {
+        r1 = Array<double>({n, n});
+        r2 = Array<double>({n, n, n, n});
+    }
+    pyci_wfn *dw = wfn.device();
+    {
+        py::gil_scoped_release nogil;
+        check(pyci_compute_rdms(dw, coeffs.data(), r1.mutable_data(), r2.mutable_data()));
+    }
+    return py::make_tuple(r1, r2);
+}
+
+}  // namespace ph
+
+using namespace ph;
+
+PYBIND11_MODULE(_pyci, m) {
+    py::options options;
+    options.disable_function_signatures();
+    m.doc() = "pyci_amd: MI355X (gfx950) configuration-interaction engine behind PyCI's _pyci API";
+    m.attr("__version__") = "0.6.1+mi355x";
+    m.attr("c_long") = py::dtype::of<int64_t>();
+    m.attr("c_ulong") = py::dtype::of<uint64_t>();
+    m.attr("c_double") = py::dtype::of<double>();
+
+    py::register_exception_translator([](std::exception_ptr p) {
+        try {
+            if (p) std::rethrow_exception(p);
+        } catch (const EngineTypeError &e) {
+            PyErr_SetString(PyExc_TypeError, e.what());
+        }
+    });
+
+    // ---- integrals ----
+    py::class_<SQuantOp> sq(m, "secondquant_op");
+    sq.def_readonly("nbasis", &SQuantOp::nbasis)
+        .def_readonly("ecore", &SQuantOp::ecore)
+        .def_readonly("one_mo", &SQuantOp::one_mo)
+        .def_readonly("two_mo", &SQuantOp::two_mo)
+        .def_readonly("h", &SQuantOp::h)
+        .def_readonly("v", &SQuantOp::v)
+        .def_readonly("w", &SQuantOp::w)
+        .def(py::init<const std::string &>(), py::arg("filename"))
+        .def(py::init<double, Array<double>, Array<double>>(), py::arg("ecore"), py::arg("one_mo"), py::arg("two_mo"))
+        .def("to_file", &SQuantOp::to_file, py::arg("filename"), py::arg("nelec") = 0, py::arg("ms2") = 0,
+             py::arg("tol") = 0.0);
+
+    // ---- wave functions ----
+    py::class_<Wfn> wf(m, "wavefunction");
+    wf.def_readonly("nbasis", &Wfn::nbasis)
+        .def_readonly("nocc", &Wfn::nocc)
+        .def_readonly("nocc_up", &Wfn::nocc_up)
+        .def_readonly("nocc_dn", &Wfn::nocc_dn)
+        .def_readonly("nvir", &Wfn::nvir)
+        .def_readonly("nvir_up", &Wfn::nvir_up)
+        .def_readonly("nvir_dn", &Wfn::nvir_dn)
+        .def("__len__", &Wfn::length)
+        .def("squeeze", &Wfn::squeeze);
+
+    auto common = [](auto &cls) {
+        using C = typename std::remove_reference_t<decltype(cls)>::type;
+        cls.def("__getitem__", [](const C &w, i64 i) { return getitem(w, i); }, py::arg("index"))
+            .def("to_file", [](const C &w, const std::string &f) { w.to_file(f); }, py::arg("filename"))
+            .def("to_det_array", [](const C &w, i64 lo, i64 hi) { return to_det_array(w, lo, hi); },
+                 py::arg("low") = -1, py::arg("high") = -1)
+            .def("to_occ_array", [](const C &w, i64 lo, i64 hi) { return to_occ_array(w, lo, hi); },
+                 py::arg("low") = -1, py::arg("high") = -1)
+            .def("index_det", [](const C &w, Array<u64> d) { return w.index_det(det_arg(w, d)); }, py::arg("det"))
+            .def("index_det_from_rank",
+                 [](const C &w, std::pair<u64, u64> r) { return w.index_det_from_rank(r); }, py::arg("rank"))
+            .def("rank_det", [](const C &w, Array<u64> d) { return w.rank_det(det_arg(w, d)); }, py::arg("det"))
+            .def("add_det", [](C &w, Array<u64> d) { return w.add_det(det_arg(w, d)); }, py::arg("det"))
+            .def("add_occs",
+                 [](C &w, Array<i64> o) {
+                     const i64 need = (w.kind == PYCI_WFN_FULLCI) ? w.nocc_up + w.nocc_dn : w.nocc_up;
+                     if ((i64)o.size() < need) throw py::value_error("occs has too few entries");
+                     return w.add_det_from_occs(o.data());
+                 },
+                 py::arg("occs"))
+            .def("add_hartreefock_det", &C::add_hartreefock_det)
+            .def("add_all_dets", [](C &w, i64) { w.fill_all_dets(); }, py::arg("nthread") = -1)
+            .def("add_excited_dets", &C::py_add_excited_dets, py::arg("exc"), py::arg("ref") = py::none())
+            .def("add_dets_from_wfn", [](C &w, const C &o) { add_dets_from(w, o); }, py::arg("wfn"))
+            .def("reserve", [](C &w, i64 n) { w.reserve(n); }, py::arg("n"));
+    };
+    py::class_<OneSpinWfn, Wfn> one(m, "one_spin_wfn");
+    common(one);
+    py::class_<TwoSpinWfn, Wfn> two(m, "two_spin_wfn");
+    common(two);
+
+    py::class_<DOCIWfn, OneSpinWfn> doci(m, "doci_wfn");
+    doci.def(py::init([](const DOCIWfn &o) {
+                 auto w = std::make_unique<DOCIWfn>(o.nbasis, o.nocc_up, o.nocc_dn);
+                 copy_from(*w, o);
+                 return w;
+             }),
+             py::arg("wfn"))
+        .def(py::init([](const std::string &f) {
+                 auto w = from_file<DOCIWfn>(f);
+                 if (w->nocc_up != w->nocc_dn) throw std::invalid_argument("nocc_up != nocc_dn");
+                 w->kind = PYCI_WFN_DOCI;
+                 return w;
+             }),
+             py::arg("filename"))
+        .def(py::init<i64, i64, i64>(), py::arg("nbasis"), py::arg("nocc_up"), py::arg("nocc_dn"))
+        .def(py::init(&from_dets<DOCIWfn>), py::arg("nbasis"), py::arg("nocc_up"), py::arg("nocc_dn"),
+             py::arg("array"))
+        .def(py::init(&from_occs<DOCIWfn>), py::arg("nbasis"), py::arg("nocc_up"), py::arg("nocc_dn"),
+             py::arg("array"));
+
+    py::class_<FullCIWfn, TwoSpinWfn> fci(m, "fullci_wfn");
+    fci.def(py::init(&fullci_from_doci), py::arg("wfn"))
+        .def(py::init([](const FullCIWfn &o) {
+                 auto w = std::make_unique<FullCIWfn>(o.nbasis, o.nocc_up, o.nocc_dn);
+                 copy_from(*w, o);
+                 return w;
+             }),
+             py::arg("wfn"))
+        .def(py::init(&from_file<FullCIWfn>), py::arg("filename"))
+        .def(py::init<i64, i64, i64>(), py::arg("nbasis"), py::arg("nocc_up"), py::arg("nocc_dn"))
+        .def(py::init(&from_dets<FullCIWfn>), py::arg("nbasis"), py::arg("nocc_up"), py::arg("nocc_dn"),
+             py::arg("array"))
+        .def(py::init(&from_occs<FullCIWfn>), py::arg("nbasis"), py::arg("nocc_up"), py::arg("nocc_dn"),
+             py::arg("array"));
+
+    py::class_<GenCIWfn, OneSpinWfn> gen(m, "genci_wfn");
+    gen.def(py::init([](const DOCIWfn &d) { return genci_from_fullci(*fullci_from_doci(d)); }), py::arg("wfn"))
+        .def(py::init([](const FullCIWfn &f) { return genci_from_fullci(f); }), py::arg("wfn"))
+        .def(py::init([](const GenCIWfn &o) {
+                 auto w = std::make_unique<GenCIWfn>(o.nbasis, o.nocc_up, o.nocc_dn);
+                 copy_from(*w, o);
+                 return w;
+             }),
+             py::arg("wfn"))
+        .def(py::init([](const std::string &f) {
+                 auto w = from_file<GenCIWfn>(f);
+                 if (w->nocc_dn) throw std::invalid_argument("nocc_dn != 0");
+                 return w;
+             }),
+             py::arg("filename"))
+        .def(py::init<i64, i64, i64>(), py::arg("nbasis"), py::arg("nocc_up"), py::arg("nocc_dn"))
+        .def(py::init(&from_dets<GenCIWfn>), py::arg("nbasis"), py::arg("nocc_up"), py::arg("nocc_dn"),
+             py::arg("array"))
+        .def(py::init(&from_occs<GenCIWfn>), py::arg("nbasis"), py::arg("nocc_up"), py::arg("nocc_dn"),
+             py::arg("array"));
+
+    // ---- sparse operator ----
+    py::class_<SparseOp> so(m, "sparse_op");
+    so.def_readonly("ecore", &SparseOp::ecore)
+        .def_readonly("symmetric", &SparseOp::symmetric)
+        .def_readonly("size", &SparseOp::size)
+        .def_readonly("shape", &SparseOp::shape)
+        .def_property_readonly("dtype", [](const SparseOp &) { return py::dtype::of<double>(); });
+    so.def(py::init([](SQuantOp &h, DOCIWfn &w, i64 r, i64 c, bool s) { return new SparseOp(h, w, r, c, s); }),
+           py::arg("ham"), py::arg("wfn"), py::arg("nrow") = -1, py::arg("ncol") = -1, py::arg("symmetric") = true)
+        .def(py::init([](SQuantOp &h, FullCIWfn &w, i64 r, i64 c, bool s) { return new SparseOp(h, w, r, c, s); }),
+             py::arg("ham"), py::arg("wfn"), py::arg("nrow") = -1, py::arg("ncol") = -1, py::arg("symmetric") = true)
+        .def(py::init([](SQuantOp &h, GenCIWfn &w, i64 r, i64 c, bool s) { return new SparseOp(h, w, r, c, s); }),
+             py::arg("ham"), py::arg("wfn"), py::arg("nrow") = -1, py::arg("ncol") = -1, py::arg("symmetric") = true)
+        .def("update", [](SparseOp &o, SQuantOp &h, DOCIWfn &w) { o.update(h, w); }, py::arg("ham"), py::arg("wfn"))
+        .def("update", [](SparseOp &o, SQuantOp &h, FullCIWfn &w) { o.update(h, w); }, py::arg("ham"), py::arg("wfn"))
+        .def("update", [](SparseOp &o, SQuantOp &h, GenCIWfn &w) { o.update(h, w); }, py::arg("ham"), py::arg("wfn"))
+        .def("__call__", &SparseOp::matvec, py::arg("x"))
+        .def("__call__", &SparseOp::matvec_out, py::arg("x"), py::arg("out"))
+        .def("matvec", &SparseOp::matvec, py::arg("x"))
+        .def("matvec", &SparseOp::matvec_out, py::arg("x"), py::arg("out"))
+        .def("get_element", &SparseOp::get_element, py::arg("i"), py::arg("j"))
+        .def("solve", &SparseOp::solve, py::arg("n") = 1, py::arg("c0") = py::none(), py::arg("ncv") = -1,
+             py::arg("maxiter") = -1, py::arg("tol") = 1.0e-12, py::arg("precision") = "f64")
+        .def("reserve", [](SparseOp &, i64) {}, py::arg("n"))
+        .def("squeeze", [](SparseOp &) {})
+        .def("data", [](const SparseOp &o) { return o.export_csr()[0]; })
+        .def("indices", [](const SparseOp &o) { return o.export_csr()[1]; })
+        .def("indptr", [](const SparseOp &o) { return o.export_csr()[2]; })
+        .def("export", &SparseOp::export_csr)
+        .def("diagonal", &SparseOp::diagonal);
+
+    // ---- free functions ----
+    m.def("get_num_threads", [] { return g_threads; });
+    m.def("set_num_threads", [](i64 n) { g_threads = std::max<i64>(n, 1); }, py::arg("n"));
+    m.def("popcnt",
+          [](Array<u64> d) {
+              i64 c = 0;
+              for (i64 i = 0; i < (i64)d.size(); ++i) c += __builtin_popcountll(d.data()[i]);
+              return c;
+          },
+          py::arg("det"));
+    m.def("ctz",
+          [](Array<u64> d) {
+              for (i64 i = 0; i < (i64)d.size(); ++i)
+                  if (d.data()[i]) return (i64)__builtin_ctzll(d.data()[i]) + 64 * i;
+              return (i64)0;
+          },
+          py::arg("det"));
+    m.def("add_hci", &add_hci<DOCIWfn>, py::arg("ham"), py::arg("wfn"), py::arg("coeffs"), py::arg("eps") = 1.0e-5,
+          py::arg("nthread") = -1);
+    m.def("add_hci", &add_hci<FullCIWfn>, py::arg("ham"), py::arg("wfn"), py::arg("coeffs"), py::arg("eps") = 1.0e-5,
+          py::arg("nthread") = -1);
+    m.def("add_hci", &add_hci<GenCIWfn>, py::arg("ham"), py::arg("wfn"), py::arg("coeffs"), py::arg("eps") = 1.0e-5,
+          py::arg("nthread") = -1);
+    m.def("compute_rdms", [](DOCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));
+    m.def("compute_rdms", [](FullCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));
+    m.def("compute_rdms", [](GenCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));
+    m.def("device_count", [] {
+        int c = 0;
+        check(pyci_device_count(&c));
+        return c;
+    });
+}

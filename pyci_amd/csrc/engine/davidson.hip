@@ -1,0 +1,507 @@
+// Block Davidson eigensolver on the GPU.
+//
+// Reference: SparseOp::solve_ci (sparseop.cpp:114-146) runs Spectra's implicitly
+// restarted Lanczos (SymEigsSolver, SmallestAlge) on the host. This solver keeps
+// the basis V, its image W = H V, the Ritz vectors and the corrections in HBM
+// (column-major, one vector contiguous). Per iteration:
+//   sigma     W[:, new] = H V[:, new]            (SpMM, HBM-bound; FP64 or FP32 values)
+//   project   G[:, new] = V^T W[:, new]          (GPU reduction, FP64)
+//   RR        (theta, Y) = eig(G)                (m x m, m <= ncv, host Jacobi, FP64)
+//   residual  X = V Y, R = W Y - X theta, |R|    (one fused GPU pass, FP64)
+//   correct   T = R / (theta - diag(H))          (Davidson diagonal preconditioner)
+//   orthog.   T -= V (V^T T), twice; normalise   (classical Gram-Schmidt with re-orthogonalisation)
+// A root is converged when |r| <= tol * max(|theta|, eps^(2/3)), the criterion
+// Spectra applies to its Ritz estimates. In the mixed mode the first phase uses
+// the FP32-value sigma with FP32 accumulation; it hands its Ritz vectors to an
+// FP64 phase (the residual-correction pass) that converges to the FP64 tolerance.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "internal.hpp"
+
+namespace pe {
+
+constexpr int DT = 256;
+
+// partial[b][ma][mb] = sum over rows of block b of A[:, i] * B[:, j] (8x8 tile per blockIdx.y)
+__global__ void __launch_bounds__(DT) k_gram(i64 nd, const double *A, i64 lda, int ma, const double *B, i64 ldb,
+                                             int mb, double *partial) {
+    const int tiles_j = (mb + 7) / 8;
+    const int i0 = (blockIdx.y / tiles_j) * 8, j0 = (blockIdx.y % tiles_j) * 8;
+    const int ni = min(8, ma - i0), nj = min(8, mb - j0);
+    double acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = 0.0;
+    for (i64 r = (i64)blockIdx.x * DT + threadIdx.x; r < nd; r += (i64)gridDim.x * DT) {
+        double a[8], b[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a[i] = (i < ni) ? A[(i64)(i0 + i) * lda + r] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[j] = (j < nj) ? B[(i64)(j0 + j) * ldb + r] : 0.0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[i][j] += a[i] * b[j];
+    }
+    __shared__ double red[DT / 64][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            double v = acc[i][j];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if (lane == 0) red[wave][i * 8 + j] = v;
+        }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int i = threadIdx.x / 8, j = threadIdx.x % 8;
+        double v = 0.0;
+        for (int w = 0; w < DT / 64; ++w) v += red[w][threadIdx.x];
+        if (i < ni && j < nj) partial[((i64)blockIdx.x * ma + i0 + i) * mb + j0 + j] = v;
+    }
+}
+
+__global__ void k_reduce_partials(const double *partial, int nblk, int count, double *out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    double v = 0.0;
+    for (int b = 0; b < nblk; ++b) v += partial[(i64)b * count + t];
+    out[t] = v;
+}
+
+// T[:, j] += alpha * V[:, 0:m] C[:, j]   (C is m x k column-major on the device)
+__global__ void k_gemm_update(i64 nd, const double *V, i64 ldv, int m, const double *C, int k, double alpha,
+                              double *T, i64 ldt) {
+    for (i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x; r < nd; r += (i64)gridDim.x * blockDim.x) {
+        for (int j = 0; j < k; ++j) {
+            double acc = 0.0;
+            for (int i = 0; i < m; ++i) acc += V[(i64)i * ldv + r] * C[j * m + i];
+            T[(i64)j * ldt + r] += alpha * acc;
+        }
+    }
+}
+
+// Ritz vectors, images, preconditioned residuals and residual norm partials.
+__global__ void __launch_bounds__(DT) k_ritz(i64 nd, const double *V, const double *W, i64 ld, int m,
+                                             const double *Y, const double *theta, int k, const double *diag,
+                                             double *X, double *WX, double *T, double *npart) {
+    double ss[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) ss[j] = 0.0;
+    for (i64 r = (i64)blockIdx.x * DT + threadIdx.x; r < nd; r += (i64)gridDim.x * DT) {
+        double x[16], w[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x[j] = w[j] = 0.0;
+        for (int i = 0; i < m; ++i) {
+            const double v = V[(i64)i * ld + r], ww = W[(i64)i * ld + r];
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (j < k) {
+                    const double y = Y[j * m + i];
+                    x[j] += v * y;
+                    w[j] += ww * y;
+                }
+        }
+        const double d = diag[r];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (j < k) {
+                const double res = w[j] - theta[j] * x[j];
+                double den = theta[j] - d;
+                if (fabs(den) < 1e-8) den = (den < 0.0) ? -1e-8 : 1e-8;
+                X[(i64)j * ld + r] = x[j];
+                WX[(i64)j * ld + r] = w[j];
+                T[(i64)j * ld + r] = res / den;
+                ss[j] += res * res;
+            }
+    }
+    __shared__ double red[DT / 64][16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        double v = ss[j];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) red[wave][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)k) {
+        double v = 0.0;
+        for (int w = 0; w < DT / 64; ++w) v += red[w][threadIdx.x];
+        npart[(i64)blockIdx.x * k + threadIdx.x] = v;
+    }
+}
+
+__global__ void k_scale(double *x, i64 n, double s) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) x[i] *= s;
+}
+
+// col-major (nd x k, ld) <-> row-major (nd x k), with optional precision change
+template <typename S, typename D>
+__global__ void k_col2row(i64 nd, int k, const S *src, i64 ld, D *dst) {
+    for (i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x; r < nd; r += (i64)gridDim.x * blockDim.x)
+        for (int j = 0; j < k; ++j) dst[r * k + j] = (D)src[(i64)j * ld + r];
+}
+
+template <typename S, typename D>
+__global__ void k_row2col(i64 nd, int k, const S *src, D *dst, i64 ld) {
+    for (i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x; r < nd; r += (i64)gridDim.x * blockDim.x)
+        for (int j = 0; j < k; ++j) dst[(i64)j * ld + r] = (D)src[r * k + j];
+}
+
+// ---------------------------------------------------------------------------
+// Host: cyclic Jacobi eigensolver for the small symmetric Rayleigh matrix.
+// ---------------------------------------------------------------------------
+static void jacobi_eig(int m, std::vector<double> a, std::vector<double> &w, std::vector<double> &v) {
+    // a: m x m row-major symmetric; returns ascending w and eigenvectors as columns of v (row-major m x m)
+    v.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i) v[(size_t)i * m + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int p = 0; p < m; ++p)
+            for (int q = 0; q < m; ++q) {
+                double x = a[(size_t)p * m + q] * a[(size_t)p * m + q];
+                tot += x;
+                if (p != q) off += x;
+            }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int p = 0; p < m - 1; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                double apq = a[(size_t)p * m + q];
+                if (apq == 0.0) continue;
+                double app = a[(size_t)p * m + p], aqq = a[(size_t)q * m + q];
+                double tau = (aqq - app) / (2.0 * apq);
+                double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+                for (int r = 0; r < m; ++r) {
+                    double arp = a[(size_t)r * m + p], arq = a[(size_t)r * m + q];
+                    a[(size_t)r * m + p] = c * arp - s * arq;
+                    a[(size_t)r * m + q] = s * arp + c * arq;
+                }
+                for (int r = 0; r < m; ++r) {
+                    double apr = a[(size_t)p * m + r], aqr = a[(size_t)q * m + r];
+                    a[(size_t)p * m + r] = c * apr - s * aqr;
+                    a[(size_t)q * m + r] = s * apr + c * aqr;
+                }
+                for (int r = 0; r < m; ++r) {
+                    double vrp = v[(size_t)r * m + p], vrq = v[(size_t)r * m + q];
+                    v[(size_t)r * m + p] = c * vrp - s * vrq;
+                    v[(size_t)r * m + q] = s * vrp + c * vrq;
+                }
+            }
+    }
+    std::vector<int> order(m);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return a[(size_t)x * m + x] < a[(size_t)y * m + y]; });
+    std::vector<double> vv((size_t)m * m);
+    w.resize(m);
+    for (int c = 0; c < m; ++c) {
+        w[c] = a[(size_t)order[c] * m + order[c]];
+        for (int r = 0; r < m; ++r) vv[(size_t)r * m + c] = v[(size_t)r * m + order[c]];
+    }
+    v.swap(vv);
+}
+
+struct Davidson {
+    pyci_op *op;
+    hipStream_t s;
+    i64 nd;
+    int k, mmax;
+    DevBuf<double> V, W, X, WX, T, rowbuf, rowout, partial, small;
+    DevBuf<float> row32, out32;
+    std::vector<double> G;  // mmax x mmax row-major
+    int m = 0;
+    int nblk;
+    pyci_solve_stats st{};
+
+    Davidson(pyci_op *o, int k_, int mmax_) : op(o), s(o->ctx->stream), nd(o->nrow), k(k_), mmax(mmax_) {
+        V.alloc((size_t)(nd * mmax));
+        W.alloc((size_t)(nd * mmax));
+        X.alloc((size_t)(nd * k));
+        WX.alloc((size_t)(nd * k));
+        T.alloc((size_t)(nd * k));
+        rowbuf.alloc((size_t)(nd * k));
+        rowout.alloc((size_t)(nd * k));
+        nblk = (int)std::min<i64>(512, (nd + DT - 1) / DT);
+        partial.alloc((size_t)nblk * (size_t)std::max(mmax * k, 64) + 64);
+        small.alloc((size_t)(mmax * mmax + 4 * mmax + 64));
+        G.assign((size_t)mmax * mmax, 0.0);
+    }
+
+    unsigned g1() const { return (unsigned)std::min<i64>(4096, (nd + 255) / 256); }
+
+    // out (ma x mb, column-major on the device at `dst`) = A^T B
+    void gram(const double *A, int ma, const double *B, int mb, double *dst) {
+        dim3 grid((unsigned)nblk, (unsigned)(((ma + 7) / 8) * ((mb + 7) / 8)));
+        k_gram<<<grid, DT, 0, s>>>(nd, A, nd, ma, B, nd, mb, partial.p);
+        PE_LAUNCH_CHECK();
+        int cnt = ma * mb;
+        k_reduce_partials<<<(cnt + 255) / 256, 256, 0, s>>>(partial.p, nblk, cnt, dst);
+        PE_LAUNCH_CHECK();
+        // dst is [i][j] row-major (i < ma); as column-major m x k with m = ma it is
+        // transposed, so callers that need C (ma x mb col-major) use gram_cm.
+    }
+
+    // C (ma x mb column-major, on device) = A^T B
+    void gram_cm(const double *A, int ma, const double *B, int mb, double *C) {
+        gram(A, ma, B, mb, small.p + (size_t)mmax * mmax);
+        // transpose row-major [ma][mb] to column-major via host (tiny)
+        std::vector<double> h((size_t)ma * mb), t((size_t)ma * mb);
+        PE_HIP(hipMemcpyAsync(h.data(), small.p + (size_t)mmax * mmax, sizeof(double) * ma * mb,
+                              hipMemcpyDeviceToHost, s));
+        PE_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < ma; ++i)
+            for (int j = 0; j < mb; ++j) t[(size_t)j * ma + i] = h[(size_t)i * mb + j];
+        PE_HIP(hipMemcpyAsync(C, t.data(), sizeof(double) * ma * mb, hipMemcpyHostToDevice, s));
+    }
+
+    std::vector<double> gram_host(const double *A, int ma, const double *B, int mb) {
+        gram(A, ma, B, mb, small.p);
+        std::vector<double> h((size_t)ma * mb);
+        PE_HIP(hipMemcpyAsync(h.data(), small.p, sizeof(double) * ma * mb, hipMemcpyDeviceToHost, s));
+        PE_HIP(hipStreamSynchronize(s));
+        return h;  // [i][j]
+    }
+
+    // W[:, c0:c0+cnt] = H V[:, c0:c0+cnt]
+    void apply(int c0, int cnt, bool fp32) {
+        const double *src = V.p + (i64)c0 * nd;
+        double *dst = W.p + (i64)c0 * nd;
+        if (fp32) {
+            if (!row32.p || row32.n < (size_t)(nd * cnt)) {
+                row32.alloc((size_t)(nd * k));
+                out32.alloc((size_t)(nd * k));
+            }
+            k_col2row<double, float><<<g1(), 256, 0, s>>>(nd, cnt, src, nd, row32.p);
+            PE_LAUNCH_CHECK();
+            sigma_f32(op, cnt, row32.p, out32.p, s);
+            k_row2col<float, double><<<g1(), 256, 0, s>>>(nd, cnt, out32.p, dst, nd);
+            PE_LAUNCH_CHECK();
+            st.sigma_calls_f32 += cnt;
+        } else if (cnt == 1) {
+            sigma_f64(op, 1, src, dst, s);
+        } else {
+            k_col2row<double, double><<<g1(), 256, 0, s>>>(nd, cnt, src, nd, rowbuf.p);
+            PE_LAUNCH_CHECK();
+            sigma_f64(op, cnt, rowbuf.p, rowout.p, s);
+            k_row2col<double, double><<<g1(), 256, 0, s>>>(nd, cnt, rowout.p, dst, nd);
+            PE_LAUNCH_CHECK();
+        }
+        st.sigma_calls += cnt;
+    }
+
+    // Orthogonalise column V[:, m] against V[:, 0:m] twice and normalise.
+    // Returns false (column rejected) if it is numerically dependent.
+    bool orthonormalize_new(double norm0) {
+        double *col = V.p + (i64)m * nd;
+        for (int pass = 0; pass < 2 && m > 0; ++pass) {
+            double *C = small.p + (size_t)mmax * mmax + 2 * mmax;  // m coefficients
+            gram_cm(V.p, m, col, 1, C);
+            k_gemm_update<<<g1(), 256, 0, s>>>(nd, V.p, nd, m, C, 1, -1.0, col, nd);
+            PE_LAUNCH_CHECK();
+        }
+        std::vector<double> nn = gram_host(col, 1, col, 1);
+        double nrm = std::sqrt(std::max(nn[0], 0.0));
+        if (!(nrm > 1e-10 * std::max(norm0, 1e-300)) || !(nrm > 1e-300)) return false;
+        k_scale<<<g1(), 256, 0, s>>>(col, nd, 1.0 / nrm);
+        PE_LAUNCH_CHECK();
+        return true;
+    }
+
+    void update_G(int c0, int cnt) {
+        std::vector<double> h = gram_host(V.p, c0 + cnt, W.p + (i64)c0 * nd, cnt);  // [(c0+cnt)][cnt]
+        for (int i = 0; i < c0 + cnt; ++i)
+            for (int j = 0; j < cnt; ++j) {
+                double x = h[(size_t)i * cnt + j];
+                G[(size_t)i * mmax + c0 + j] = x;
+                G[(size_t)(c0 + j) * mmax + i] = x;
+            }
+        // symmetrise the new diagonal block
+        for (int a = 0; a < cnt; ++a)
+            for (int b = a + 1; b < cnt; ++b) {
+                double x = 0.5 * (G[(size_t)(c0 + a) * mmax + c0 + b] + G[(size_t)(c0 + b) * mmax + c0 + a]);
+                G[(size_t)(c0 + a) * mmax + c0 + b] = G[(size_t)(c0 + b) * mmax + c0 + a] = x;
+            }
+    }
+
+    // Runs Davidson until every root meets tol (or maxiter iterations). With
+    // fp32 set it stops early once residuals reach the FP32 floor (switch_tol).
+    // Returns true when converged to tol.
+    bool run(double tol, i64 maxiter, bool fp32, double switch_tol, std::vector<double> &theta, i64 &iters) {
+        const double eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
+        std::vector<double> best_res(k, 1e300);
+        int stall = 0;
+        for (;;) {
+            std::vector<double> gm((size_t)m * m), w, y;
+            for (int i = 0; i < m; ++i)
+                for (int j = 0; j < m; ++j) gm[(size_t)i * m + j] = G[(size_t)i * mmax + j];
+            jacobi_eig(m, gm, w, y);
+            theta.assign(w.begin(), w.begin() + k);
+            // Y (m x k col-major) and theta to the device
+            std::vector<double> yk((size_t)m * k);
+            for (int j = 0; j < k; ++j)
+                for (int i = 0; i < m; ++i) yk[(size_t)j * m + i] = y[(size_t)i * m + j];
+            double *dY = small.p, *dth = small.p + (size_t)m * k;
+            PE_HIP(hipMemcpyAsync(dY, yk.data(), sizeof(double) * m * k, hipMemcpyHostToDevice, s));
+            PE_HIP(hipMemcpyAsync(dth, theta.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+            k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, m, dY, dth, k, op->diag.p, X.p, WX.p, T.p,
+                                                partial.p);
+            PE_LAUNCH_CHECK();
+            std::vector<double> np((size_t)nblk * k);
+            PE_HIP(hipMemcpyAsync(np.data(), partial.p, sizeof(double) * nblk * k, hipMemcpyDeviceToHost, s));
+            PE_HIP(hipStreamSynchronize(s));
+            std::vector<double> res(k, 0.0);
+            for (int b = 0; b < nblk; ++b)
+                for (int j = 0; j < k; ++j) res[j] += np[(size_t)b * k + j];
+            bool all = true, floor = true;
+            for (int j = 0; j < k; ++j) {
+                res[j] = std::sqrt(res[j]);
+                if (j < 64) st.residual[j] = res[j];
+                const double lim = tol * std::max(std::fabs(theta[j]), eps23);
+                if (!(res[j] <= lim)) all = false;
+                if (!(res[j] <= std::max(lim, switch_tol * std::max(std::fabs(theta[j]), 1.0)))) floor = false;
+            }
+            ++iters;
+            ++st.iterations;
+            if (all) return true;
+            if (fp32 && floor) return false;
+            if (m >= nd) {
+                // the basis spans the whole space: the Ritz pairs are exact to rounding
+                return true;
+            }
+            // stagnation guard: residuals at the rounding floor stop improving
+            bool improved = false;
+            for (int j = 0; j < k; ++j)
+                if (res[j] < 0.9 * best_res[j]) {
+                    best_res[j] = res[j];
+                    improved = true;
+                }
+            stall = improved ? 0 : stall + 1;
+            if (stall >= (fp32 ? 8 : 25)) {
+                bool tiny = true;
+                for (int j = 0; j < k; ++j)
+                    if (!(res[j] <= 1e-7 * std::max(std::fabs(theta[j]), 1.0))) tiny = false;
+                if (fp32) return false;
+                if (tiny) return true;
+                return false;
+            }
+            if (iters >= maxiter) return false;
+            // unconverged roots contribute corrections
+            std::vector<int> add;
+            for (int j = 0; j < k; ++j)
+                if (!(res[j] <= tol * std::max(std::fabs(theta[j]), eps23))) add.push_back(j);
+            if (m + (int)add.size() > mmax) {
+                // restart on the current Ritz vectors
+                PE_HIP(hipMemcpyAsync(V.p, X.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
+                PE_HIP(hipMemcpyAsync(W.p, WX.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
+                std::fill(G.begin(), G.end(), 0.0);
+                for (int j = 0; j < k; ++j) G[(size_t)j * mmax + j] = theta[j];
+                m = k;
+            }
+            const int m0 = m;
+            for (int j : add) {
+                if (m >= mmax || m >= nd) break;
+                PE_HIP(hipMemcpyAsync(V.p + (i64)m * nd, T.p + (i64)j * nd, sizeof(double) * nd,
+                                      hipMemcpyDeviceToDevice, s));
+                std::vector<double> n0 = gram_host(V.p + (i64)m * nd, 1, V.p + (i64)m * nd, 1);
+                if (orthonormalize_new(std::sqrt(std::max(n0[0], 0.0)))) ++m;
+            }
+            if (m == m0) {
+                // no usable direction: accept if the residuals are tiny, else fail
+                bool tiny = true;
+                for (int j = 0; j < k; ++j)
+                    if (!(res[j] <= 1e-7 * std::max(std::fabs(theta[j]), 1.0))) tiny = false;
+                return tiny && !fp32;
+            }
+            apply(m0, m - m0, fp32);
+            update_G(m0, m - m0);
+        }
+    }
+};
+
+void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, double tol, int precision,
+                    double *evals, double *evecs, pyci_solve_stats *stats) {
+    const i64 nd = op->nrow;
+    hipStream_t s = op->ctx->stream;
+    const int k = (int)n;
+    PE_REQUIRE(k <= 16, PYCI_EUNSUPPORTED, "solve supports at most 16 roots per call");
+    i64 mm = (ncv > 0) ? ncv : std::min<i64>(nd, std::max<i64>(2 * n + 1, 20));
+    mm = std::max<i64>(mm, 2 * n + 1);
+    mm = std::min<i64>(mm, nd);
+    mm = std::max<i64>(mm, n);
+    const i64 maxit = (maxiter > 0) ? maxiter : std::max<i64>(1000, std::min<i64>(n * nd * 10, 100000));
+    if (precision == PYCI_PREC_MIXED) ensure_vals32(op);
+    Davidson D(op, k, (int)mm);
+    // initial block: c0 (if given) then unit vectors on the lowest diagonal
+    // entries, each mixed with 1e-2 of a fixed pseudo-random vector so that no
+    // symmetry sector is excluded.
+    std::vector<double> hd((size_t)nd);
+    PE_HIP(hipMemcpyAsync(hd.data(), op->diag.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    std::vector<i64> idx((size_t)nd);
+    std::iota(idx.begin(), idx.end(), 0);
+    const i64 want = std::min<i64>(nd, k + 1);
+    std::partial_sort(idx.begin(), idx.begin() + want, idx.end(),
+                      [&](i64 a, i64 b) { return hd[(size_t)a] < hd[(size_t)b] || (hd[(size_t)a] == hd[(size_t)b] && a < b); });
+    std::vector<double> init((size_t)nd);
+    int used = 0;
+    D.m = 0;
+    for (int j = 0; j < k; ++j) {
+        if (j == 0 && c0) {
+            std::memcpy(init.data(), c0, sizeof(double) * nd);
+        } else {
+            u64 seed = 0x243f6a8885a308d3ULL + (u64)j;
+            for (i64 r = 0; r < nd; ++r) init[(size_t)r] = 1e-2 * ((double)(mix64(seed + (u64)r) >> 11) * 0x1p-53 - 0.5);
+            init[(size_t)idx[(size_t)std::min<i64>(used, nd - 1)]] += 1.0;
+            ++used;
+        }
+        PE_HIP(hipMemcpyAsync(D.V.p + (i64)D.m * nd, init.data(), sizeof(double) * nd, hipMemcpyHostToDevice, s));
+        std::vector<double> n0 = D.gram_host(D.V.p + (i64)D.m * nd, 1, D.V.p + (i64)D.m * nd, 1);
+        if (D.orthonormalize_new(std::sqrt(std::max(n0[0], 0.0)))) ++D.m;
+    }
+    PE_REQUIRE(D.m == k, PYCI_ENOCONV, "did not converge");
+    const bool mixed = (precision == PYCI_PREC_MIXED);
+    D.apply(0, D.m, mixed);
+    D.update_G(0, D.m);
+    std::vector<double> theta;
+    i64 iters = 0;
+    bool ok;
+    if (mixed) {
+        D.run(tol, maxit, true, 2e-5, theta, iters);
+        // FP64 residual-correction phase: restart on the FP32-phase Ritz vectors
+        // with exact FP64 images.
+        PE_HIP(hipMemcpyAsync(D.V.p, D.X.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
+        D.m = 0;
+        for (int j = 0; j < k; ++j) {
+            if (j != D.m)
+                PE_HIP(hipMemcpyAsync(D.V.p + (i64)D.m * nd, D.V.p + (i64)j * nd, sizeof(double) * nd,
+                                      hipMemcpyDeviceToDevice, s));
+            if (D.orthonormalize_new(1.0)) ++D.m;
+        }
+        PE_REQUIRE(D.m == k, PYCI_ENOCONV, "did not converge");
+        std::fill(D.G.begin(), D.G.end(), 0.0);
+        D.apply(0, D.m, false);
+        D.update_G(0, D.m);
+        ok = D.run(tol, maxit, false, 0.0, theta, iters);
+    } else {
+        ok = D.run(tol, maxit, false, 0.0, theta, iters);
+    }
+    if (stats) *stats = D.st;
+    PE_REQUIRE(ok, PYCI_ENOCONV, "did not converge");
+    // reference order: descending for n > 1 (sparseop.cpp:134-144)
+    std::vector<double> hx((size_t)(nd * k));
+    PE_HIP(hipMemcpyAsync(hx.data(), D.X.p, sizeof(double) * nd * k, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < k; ++j) {
+        const int src = (k > 1) ? (k - 1 - j) : 0;
+        evals[j] = theta[src] + op->ecore;
+        std::memcpy(evecs + (i64)j * nd, hx.data() + (i64)src * nd, sizeof(double) * nd);
+    }
+}
+
+}  // namespace pe

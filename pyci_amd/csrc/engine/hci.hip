@@ -1,0 +1,336 @@
+// Heat-bath CI selection on the GPU.
+//
+// Reference: add_hci (hci.cpp:252-279) with the per-determinant screening of
+// hci.cpp:49-238. A candidate D' of source determinant D_i is added when the
+// magnitude of its matrix element exceeds eps/|c_i| and D' is not yet in the
+// wave function (singles use the full single-excitation element, 1-1 doubles
+// |<ik|ab>|, same-spin doubles |<ik|ab> - <ik|ba>|, DOCI pair hops |v_kl|).
+//
+// The reference merges per-thread hash maps, so its append order depends on the
+// hash table's iteration order. Here new determinants are appended in the order
+// of their FIRST occurrence in the serial enumeration (source det ascending,
+// then the reference's loop order inside a row), which is deterministic:
+//   pass 1 counts passing new candidates (with repeats) to size a hash set,
+//   pass 2 inserts each candidate's 64-bit key with the minimum position,
+//   pass 3 counts per row the candidates that own their key's minimum,
+//   pass 4 writes owners in row order after a scan.
+// One wavefront handles one source row; the outer (i, j) loops run uniformly
+// and lanes cover the inner excitation block, so ballot compaction preserves
+// the reference order.
+#include <algorithm>
+
+#include "internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace pe {
+
+constexpr u64 EMPTY_KEY = ~0ULL;
+
+struct NewSet {
+    u64 *keys;
+    u64 *pos;
+    u64 mask;
+    unsigned long long *unique;  // number of claimed slots
+    int *overflow;
+};
+
+__device__ inline u64 set_key(const WfnView &w, u64 a, u64 b) {
+    if (w.nkey == 2) return (u64)colex_rank(a, w.binom) * (u64)w.nsd + (u64)colex_rank(b, w.binom);
+    return a;
+}
+
+// Inserts key with position p (keeps the minimum). Claimed slots are counted.
+__device__ inline void set_insert(const NewSet &S, u64 key, u64 p) {
+    u64 h = mix64(key) & S.mask;
+    for (u64 step = 0; step <= S.mask; ++step) {
+        unsigned long long old = atomicCAS((unsigned long long *)&S.keys[h], (unsigned long long)EMPTY_KEY,
+                                           (unsigned long long)key);
+        if (old == EMPTY_KEY) atomicAdd(S.unique, 1ULL);
+        if (old == EMPTY_KEY || old == key) {
+            atomicMin((unsigned long long *)&S.pos[h], (unsigned long long)p);
+            return;
+        }
+        h = (h + 1) & S.mask;
+    }
+    atomicOr(S.overflow, 1);
+}
+
+__device__ inline u64 set_find_pos(const NewSet &S, u64 key) {
+    u64 h = mix64(key) & S.mask;
+    for (u64 step = 0; step <= S.mask; ++step) {
+        u64 k = S.keys[h];
+        if (k == key) return S.pos[h];
+        if (k == EMPTY_KEY) return ~0ULL;
+        h = (h + 1) & S.mask;
+    }
+    return ~0ULL;
+}
+
+#define GIX(p, q, r, s) ((((p) * n + (q)) * n + (r)) * n + (s))
+
+// PASS 1: count, 2: insert, 3: owners per row, 4: write owners.
+template <int KIND, int PASS>
+__global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double *coeffs, double eps, i64 rows_stride,
+                                             NewSet S, unsigned long long *count, i64 *row_cnt, const i64 *row_off,
+                                             u64 *out) {
+    const int lane = threadIdx.x & 63;
+    const i64 row = (i64)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= w.ndet) return;
+    const int n = H.n;
+    const double *one = H.one, *two = H.two;
+    const u64 m = full_mask(n);
+    const u64 up = w.dets[(i64)w.nkey * row];
+    const u64 dn = (KIND == PYCI_WFN_FULLCI) ? w.dets[2 * row + 1] : 0ULL;
+    const int nou = __popcll(up), nvu = n - nou, nod = __popcll(dn), nvd = n - nod;
+    const double eps_i = eps / fabs(coeffs[row]);
+    // lane q holds the q-th occupied / virtual orbital of each string
+    int tmp;
+    auto nth = [](u64 x, int t) {
+        for (int s = 0; s < t && x; ++s) x &= x - 1;
+        return x ? __builtin_ctzll(x) : 0;
+    };
+    const int packed = nth(up, lane) | (nth(~up & m, lane) << 8) | (nth(dn, lane) << 16) | (nth(~dn & m, lane) << 24);
+    (void)tmp;
+    u64 local = 0;                 // PASS 1 count
+    int owned = 0;                 // PASS 3/4 running count in this row
+    i64 obase = (PASS == 4) ? row_off[row] : 0;
+    i64 e_base = 0;                // running position inside the row (reference order)
+
+    // Evaluates one candidate; returns true when it passes and is new.
+    auto consider = [&](bool valid, u64 nu, u64 nd, double val, u64 pos) {
+        bool pass = false;
+        u64 key = 0;
+        if (valid && fabs(val) > eps_i) {
+            if (wfn_lookup(w, nu, nd) < 0) {
+                pass = true;
+                key = set_key(w, nu, nd);
+            }
+        }
+        if (PASS == 1) {
+            local += __popcll(__ballot(pass));
+        } else if (PASS == 2) {
+            if (pass) set_insert(S, key, pos);
+        } else {
+            const bool own = pass && (set_find_pos(S, key) == pos);
+            const u64 b = __ballot(own);
+            if (PASS == 4 && own) {
+                const i64 o = obase + owned + __popcll(b & ((1ULL << lane) - 1ULL));
+                out[(i64)w.nkey * o] = nu;
+                if (w.nkey == 2) out[2 * o + 1] = nd;
+            }
+            owned += __popcll(b);
+        }
+    };
+
+    if (KIND == PYCI_WFN_DOCI) {
+        const int total = nou * nvu;
+        for (int e0 = 0; e0 < total; e0 += 64) {
+            const int e = e0 + lane;
+            const bool valid = e < total;
+            const int i = valid ? e / nvu : 0, j = valid ? e - (e / nvu) * nvu : 0;
+            const int pi = __shfl(packed, i), pj = __shfl(packed, j);
+            const int k = pi & 255, l = (pj >> 8) & 255;
+            double val = valid ? two[GIX(k, k, l, l)] : 0.0;
+            consider(valid, up ^ (1ULL << k) ^ (1ULL << l), 0ULL, val, (u64)row * (u64)rows_stride + (u64)e);
+        }
+    } else {
+        // alpha (or only) string: for i, for j: single, [1-1 block], same-spin doubles k > i, l > j
+        for (int i = 0; i < nou; ++i) {
+            for (int j = 0; j < nvu; ++j) {
+                const int ii = __shfl(packed, i) & 255, jj = (__shfl(packed, j) >> 8) & 255;
+                const int nsd = (KIND == PYCI_WFN_FULLCI) ? nod * nvd : 0;
+                const int ndbl = (nou - 1 - i) * (nvu - 1 - j);
+                const int bsize = 1 + nsd + ndbl;
+                for (int t0 = 0; t0 < bsize; t0 += 64) {
+                    const int t = t0 + lane;
+                    const bool valid = t < bsize;
+                    int k = 0, l = 0, cls = 0;
+                    if (valid && t > 0) {
+                        if (t < 1 + nsd) {
+                            cls = 1;
+                            k = (t - 1) / nvd;
+                            l = (t - 1) - k * nvd;
+                        } else {
+                            cls = 2;
+                            const int u = t - 1 - nsd, w2 = nvu - 1 - j;
+                            k = i + 1 + u / w2;
+                            l = j + 1 + (u - (u / w2) * w2);
+                        }
+                    }
+                    const int pk = __shfl(packed, k), pl = __shfl(packed, l);
+                    u64 nu = up ^ (1ULL << ii) ^ (1ULL << jj), nd = dn;
+                    double val = 0.0;
+                    if (valid) {
+                        if (cls == 0) {
+                            double v = one[ii * n + jj];
+                            for (u64 x = up; x; x &= x - 1) {
+                                int q = __builtin_ctzll(x);
+                                v += two[GIX(ii, q, jj, q)] - two[GIX(ii, q, q, jj)];
+                            }
+                            if (KIND == PYCI_WFN_FULLCI)
+                                for (u64 x = dn; x; x &= x - 1) {
+                                    int q = __builtin_ctzll(x);
+                                    v += two[GIX(ii, q, jj, q)];
+                                }
+                            val = v;
+                        } else if (cls == 1) {
+                            const int kk = (pk >> 16) & 255, ll = (pl >> 24) & 255;
+                            nd = dn ^ (1ULL << kk) ^ (1ULL << ll);
+                            val = two[GIX(ii, kk, jj, ll)];
+                        } else {
+                            const int kk = pk & 255, ll = (pl >> 8) & 255;
+                            nu ^= (1ULL << kk) ^ (1ULL << ll);
+                            val = two[GIX(ii, kk, jj, ll)] - two[GIX(ii, kk, ll, jj)];
+                        }
+                    }
+                    consider(valid, nu, nd, val, (u64)row * (u64)rows_stride + (u64)(e_base + t));
+                }
+                e_base += bsize;
+            }
+        }
+        if (KIND == PYCI_WFN_FULLCI) {
+            for (int i = 0; i < nod; ++i) {
+                for (int j = 0; j < nvd; ++j) {
+                    const int ii = (__shfl(packed, i) >> 16) & 255, jj = (__shfl(packed, j) >> 24) & 255;
+                    const int ndbl = (nod - 1 - i) * (nvd - 1 - j);
+                    const int bsize = 1 + ndbl;
+                    for (int t0 = 0; t0 < bsize; t0 += 64) {
+                        const int t = t0 + lane;
+                        const bool valid = t < bsize;
+                        int k = 0, l = 0;
+                        if (valid && t > 0) {
+                            const int u = t - 1, w2 = nvd - 1 - j;
+                            k = i + 1 + u / w2;
+                            l = j + 1 + (u - (u / w2) * w2);
+                        }
+                        const int pk = __shfl(packed, k), pl = __shfl(packed, l);
+                        u64 nd = dn ^ (1ULL << ii) ^ (1ULL << jj);
+                        double val = 0.0;
+                        if (valid) {
+                            if (t == 0) {
+                                double v = one[ii * n + jj];
+                                for (u64 x = up; x; x &= x - 1) {
+                                    int q = __builtin_ctzll(x);
+                                    v += two[GIX(ii, q, jj, q)];
+                                }
+                                for (u64 x = dn; x; x &= x - 1) {
+                                    int q = __builtin_ctzll(x);
+                                    v += two[GIX(ii, q, jj, q)] - two[GIX(ii, q, q, jj)];
+                                }
+                                val = v;
+                            } else {
+                                const int kk = (pk >> 16) & 255, ll = (pl >> 24) & 255;
+                                nd ^= (1ULL << kk) ^ (1ULL << ll);
+                                val = two[GIX(ii, kk, jj, ll)] - two[GIX(ii, kk, ll, jj)];
+                            }
+                        }
+                        consider(valid, up, nd, val, (u64)row * (u64)rows_stride + (u64)(e_base + t));
+                    }
+                    e_base += bsize;
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        if (PASS == 1) atomicAdd(count, (unsigned long long)local);
+        if (PASS == 3) row_cnt[row] = owned;
+    }
+}
+
+__global__ void k_set_clear(u64 *keys, u64 *pos, i64 n) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        keys[i] = EMPTY_KEY;
+        pos[i] = ~0ULL;
+    }
+}
+
+template <int PASS>
+static void launch_hci(int kind, const WfnView &w, const HamView &H, const double *c, double eps, i64 stride,
+                       const NewSet &S, unsigned long long *count, i64 *rc, const i64 *ro, u64 *out, hipStream_t s) {
+    unsigned g = grid_for(w.ndet, 4);
+    if (kind == PYCI_WFN_DOCI)
+        k_hci<PYCI_WFN_DOCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, stride, S, count, rc, ro, out);
+    else if (kind == PYCI_WFN_FULLCI)
+        k_hci<PYCI_WFN_FULLCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, stride, S, count, rc, ro, out);
+    else
+        k_hci<PYCI_WFN_GENCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, stride, S, count, rc, ro, out);
+    PE_LAUNCH_CHECK();
+}
+
+void wfn_index_append(pyci_wfn *w, i64 first);
+
+i64 add_hci_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, double eps) {
+    pyci_ctx *ctx = wfn->ctx;
+    hipStream_t s = ctx->stream;
+    const i64 nd = wfn->ndet;
+    if (nd == 0) return 0;
+    PE_REQUIRE(ham->nbasis == wfn->nbasis, PYCI_EINVAL, "operator and wave function have different nbasis");
+    if (wfn->kind == PYCI_WFN_FULLCI) {
+        long double tot = (long double)binom_host(wfn->nbasis, wfn->nup) * (long double)binom_host(wfn->nbasis, wfn->ndn);
+        PE_REQUIRE(tot < 9.0e18L, PYCI_EUNSUPPORTED, "determinant space too large for the GPU selection keys");
+    }
+    // positions: row * stride + in-row index; stride >= candidates per row
+    const i64 no = wfn->nup, nv = wfn->nbasis - wfn->nup, ndo = wfn->ndn, ndv = wfn->nbasis - wfn->ndn;
+    i64 stride = no * nv + no * nv * ndo * ndv + (no * (no - 1) / 2) * (nv * (nv - 1) / 2) + ndo * ndv +
+                 (ndo * (ndo - 1) / 2) * (ndv * (ndv - 1) / 2) + 1;
+    PE_REQUIRE((long double)stride * (long double)nd < 1.8e19L, PYCI_EUNSUPPORTED, "selection too large");
+    DevBuf<double> dc((size_t)nd);
+    PE_HIP(hipMemcpyAsync(dc.p, coeffs_host, sizeof(double) * nd, hipMemcpyHostToDevice, s));
+    WfnView w = view_of(wfn);
+    HamView H = view_of(ham);
+    DevBuf<unsigned long long> cnt(2);
+    DevBuf<int> ovf(1);
+    PE_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    NewSet S{nullptr, nullptr, 0, cnt.p + 1, ovf.p};
+    launch_hci<1>(wfn->kind, w, H, dc.p, eps, stride, S, cnt.p, nullptr, nullptr, nullptr, s);
+    unsigned long long upper = 0;
+    PE_HIP(hipMemcpyAsync(&upper, cnt.p, sizeof(upper), hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    if (upper == 0) return 0;
+    // hash set capacity: 2x the candidate count, capped; grown on overflow
+    i64 cap = 1024;
+    const i64 want = (i64)std::min<unsigned long long>(upper, 1ULL << 29);
+    while (cap < 2 * want) cap <<= 1;
+    DevBuf<u64> keys, pos;
+    for (;;) {
+        keys.alloc((size_t)cap);
+        pos.alloc((size_t)cap);
+        k_set_clear<<<4096, 256, 0, s>>>(keys.p, pos.p, cap);
+        PE_LAUNCH_CHECK();
+        PE_HIP(hipMemsetAsync(cnt.p + 1, 0, sizeof(unsigned long long), s));
+        PE_HIP(hipMemsetAsync(ovf.p, 0, sizeof(int), s));
+        S = NewSet{keys.p, pos.p, (u64)(cap - 1), cnt.p + 1, ovf.p};
+        launch_hci<2>(wfn->kind, w, H, dc.p, eps, stride, S, nullptr, nullptr, nullptr, nullptr, s);
+        unsigned long long uniq = 0;
+        int of = 0;
+        PE_HIP(hipMemcpyAsync(&uniq, cnt.p + 1, sizeof(uniq), hipMemcpyDeviceToHost, s));
+        PE_HIP(hipMemcpyAsync(&of, ovf.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        PE_HIP(hipStreamSynchronize(s));
+        if (!of && 2 * (i64)uniq <= cap) break;
+        PE_REQUIRE(cap < ((i64)1 << 34), PYCI_ENOMEM, "selection hash set too large");
+        cap <<= 2;
+    }
+    DevBuf<i64> rc((size_t)(nd + 1));
+    PE_HIP(hipMemsetAsync(rc.p, 0, sizeof(i64) * (nd + 1), s));
+    launch_hci<3>(wfn->kind, w, H, dc.p, eps, stride, S, nullptr, rc.p, nullptr, nullptr, s);
+    const i64 nnew = exclusive_scan(ctx, rc.p, nd + 1);
+    if (nnew == 0) return 0;
+    // grow the device det array, then write the new determinants in place
+    const i64 need = (nd + nnew) * wfn->nkey;
+    if ((size_t)need > wfn->dets.n) {
+        DevBuf<u64> nb((size_t)(need + need / 2));
+        PE_HIP(hipMemcpyAsync(nb.p, wfn->dets.p, sizeof(u64) * nd * wfn->nkey, hipMemcpyDeviceToDevice, s));
+        PE_HIP(hipStreamSynchronize(s));
+        wfn->dets = std::move(nb);
+        w = view_of(wfn);
+    }
+    launch_hci<4>(wfn->kind, w, H, dc.p, eps, stride, S, nullptr, nullptr, rc.p, wfn->dets.p + nd * wfn->nkey, s);
+    PE_HIP(hipStreamSynchronize(s));
+    wfn->ndet = nd + nnew;
+    wfn_index_append(wfn, nd);
+    PE_HIP(hipStreamSynchronize(s));
+    return nnew;
+}
+
+}  // namespace pe

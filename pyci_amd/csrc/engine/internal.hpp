@@ -1,0 +1,250 @@
+// Internal definitions shared by the engine translation units (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pyci_engine.h"
+
+namespace pe {
+
+using u64 = uint64_t;
+using i64 = int64_t;
+
+// ---------------------------------------------------------------------------
+// Errors: host code throws EngineError; the C ABI converts it into a status.
+// ---------------------------------------------------------------------------
+struct EngineError : std::runtime_error {
+    int code;
+    EngineError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_error(const std::string &msg);
+
+#define PE_HIP(x)                                                                                      \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::pe::EngineError(PYCI_EHIP, std::string(#x " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define PE_LAUNCH_CHECK() PE_HIP(hipGetLastError())
+
+#define PE_REQUIRE(cond, code, msg)                     \
+    do {                                                \
+        if (!(cond)) throw ::pe::EngineError(code, msg); \
+    } while (0)
+
+// Device buffer owned by RAII.
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr, o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p, n = o.n;
+            o.p = nullptr, o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw EngineError(e == hipErrorOutOfMemory ? PYCI_ENOMEM : PYCI_EHIP,
+                              "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " +
+                                  hipGetErrorString(e));
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace pe
+
+// ---------------------------------------------------------------------------
+// Handles
+// ---------------------------------------------------------------------------
+struct pyci_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    pe::DevBuf<pe::i64> binom;  // C(n, k) for 0 <= n, k <= 64, row-major [n][k] (65 x 65)
+};
+
+struct pyci_ham {
+    pyci_ctx *ctx = nullptr;
+    pe::i64 nbasis = 0;
+    double ecore = 0.0;
+    pe::DevBuf<double> one_mo, two_mo;
+};
+
+struct pyci_wfn {
+    pyci_ctx *ctx = nullptr;
+    int kind = 0;
+    pe::i64 nbasis = 0, nup = 0, ndn = 0;
+    int nkey = 1;  // words per determinant
+    pe::i64 ndet = 0;
+    pe::DevBuf<pe::u64> dets;  // capacity >= ndet * nkey
+    // index: perfect colex rank when the set is the full space in colex order
+    bool colex = false;
+    pe::i64 nsd = 1;  // beta (or only) string count, for FullCI colex rank
+    // otherwise: open-addressing hash (key words per slot = 2, value = index, -1 empty)
+    pe::DevBuf<pe::u64> hkeys;
+    pe::DevBuf<pe::i64> hvals;
+    pe::i64 hcap = 0;
+};
+
+struct pyci_op {
+    pyci_ctx *ctx = nullptr;
+    int kind = 0;
+    pe::i64 nrow = 0, ncol = 0;
+    int symmetric = 1;
+    double ecore = 0.0;
+    pe::i64 nnz = 0;       // stored (full-pattern) nonzeros
+    pe::i64 size_ref = 0;  // reference-format nonzeros (lower+diag if symmetric)
+    pe::DevBuf<pe::i64> rowptr;   // nrow+1
+    pe::DevBuf<pe::i64> lowcnt;   // per-row count of entries with col <= row (symmetric export)
+    pe::DevBuf<int32_t> cols;     // nnz
+    pe::DevBuf<double> vals;      // nnz
+    pe::DevBuf<float> vals32;     // nnz, built on demand for the FP32 sigma
+    pe::DevBuf<double> diag;      // nrow
+    // rows built in segments for non-symmetric update(): [row0, row1) with its ncol
+    std::vector<std::pair<pe::i64, pe::i64>> segments;
+};
+
+namespace pe {
+
+// ---------------------------------------------------------------------------
+// Device views
+// ---------------------------------------------------------------------------
+struct WfnView {
+    int kind, nkey;
+    int nb, nup, ndn;  // nup = nocc for one-spin kinds
+    i64 ndet;
+    const u64 *dets;
+    int colex;
+    i64 nsd;
+    const i64 *binom;  // 65 x 65
+    const u64 *hkeys;
+    const i64 *hvals;
+    u64 hmask;
+};
+
+struct HamView {
+    int n;
+    const double *one, *two;  // one (n,n), two (n,n,n,n)
+};
+
+WfnView view_of(const pyci_wfn *w);
+HamView view_of(const pyci_ham *h);
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+__host__ __device__ inline u64 mix64(u64 x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return x;
+}
+
+__host__ __device__ inline u64 hash_key(u64 a, u64 b) {
+    return mix64(mix64(a + 0x9e3779b97f4a7c15ULL) ^ (b + 0x632be59bd9b4e019ULL));
+}
+
+__device__ inline i64 colex_rank(u64 det, const i64 *binom) {
+    i64 r = 0;
+    int k = 0;
+    while (det) {
+        int p = __builtin_ctzll(det);
+        ++k;
+        r += binom[p * 65 + k];
+        det &= det - 1;
+    }
+    return r;
+}
+
+// Index of determinant (a, b) (b = 0 for one-spin kinds), or -1.
+__device__ inline i64 wfn_lookup(const WfnView &w, u64 a, u64 b) {
+    if (w.colex) {
+        if (w.nkey == 2) return colex_rank(a, w.binom) * w.nsd + colex_rank(b, w.binom);
+        return colex_rank(a, w.binom);
+    }
+    u64 h = hash_key(a, b) & w.hmask;
+    for (;;) {
+        i64 v = w.hvals[h];
+        if (v < 0) return -1;
+        if (w.hkeys[2 * h] == a && w.hkeys[2 * h + 1] == b) return v;
+        h = (h + 1) & w.hmask;
+    }
+}
+
+// Number of set bits of det strictly between lo and hi (lo < hi < 64).
+__device__ inline int bits_between(u64 det, int lo, int hi) {
+    u64 below = (hi >= 64) ? ~0ULL : ((1ULL << hi) - 1ULL);
+    u64 above = (lo >= 63) ? 0ULL : ~((1ULL << (lo + 1)) - 1ULL);
+    return __popcll(det & below & above);
+}
+
+// phase_single_det (common.cpp:163-194).
+__device__ inline int phase1(u64 det, int i, int a) {
+    return (bits_between(det, min(i, a), max(i, a)) & 1) ? -1 : 1;
+}
+
+// phase_double_det (common.cpp:196-261), counts on the original determinant.
+__device__ inline int phase2(u64 det, int i1, int i2, int a1, int a2) {
+    int n = bits_between(det, min(i1, a1), max(i1, a1)) + bits_between(det, min(i2, a2), max(i2, a2));
+    if (i2 < a1 || i1 > a2) ++n;
+    return (n & 1) ? -1 : 1;
+}
+
+__device__ inline u64 full_mask(int nb) { return nb >= 64 ? ~0ULL : ((1ULL << nb) - 1ULL); }
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+i64 binom_host(i64 n, i64 k);
+inline unsigned grid_for(i64 n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
+
+// Exclusive scan of n int64 counts in place on the stream; returns the total.
+i64 exclusive_scan(pyci_ctx *ctx, i64 *d_data, i64 n);
+
+// CSR build entry points (csr_build.hip)
+void build_rows(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn, i64 row0, i64 row1, i64 ncol,
+                DevBuf<i64> &rowptr_seg, DevBuf<int32_t> &cols, DevBuf<double> &vals, DevBuf<i64> &lowcnt_seg,
+                DevBuf<double> &diag_seg, i64 &nnz_seg, i64 &low_total);
+// SpMM (sigma.hip): Y = A X, X ncol x nvec row-major, Y nrow x nvec row-major
+void sigma_f64(const pyci_op *op, int nvec, const double *X, double *Y, hipStream_t s);
+void sigma_f32(const pyci_op *op, int nvec, const float *X, float *Y, hipStream_t s);
+void ensure_vals32(pyci_op *op);
+// Davidson (davidson.hip)
+void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, double tol, int precision,
+                    double *evals, double *evecs, pyci_solve_stats *stats);
+// index (index.hip)
+void wfn_build_index(pyci_wfn *w);
+void fill_all_dets_device(pyci_ctx *ctx, int kind, i64 nb, i64 nup, i64 ndn, u64 *d_out, int nword);
+// HCI / RDMs
+i64 add_hci_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, double eps);
+void rdms_device(pyci_wfn *wfn, const double *coeffs_host, double *rdm1, double *rdm2);
+
+}  // namespace pe

@@ -1,0 +1,193 @@
+// Sigma (Y = H X) over the stored full-pattern CSR: the GPU counterpart of the
+// reference's Eigen SpMV in perform_op / perform_op_symm (sparseop.cpp:96-112).
+// The symmetric operator is stored with both triangles, so one row-parallel
+// streaming pass produces (L + L^T - D) x without scatter atomics.
+//
+// Layout: X is ncol x NV and Y is nrow x NV, row-major, so the NV values a column
+// index points at share one cache line. One 64-lane wavefront per row; each lane
+// streams column indices (4 B) and values (8 B FP64 or 4 B FP32) with a stride of
+// 64 entries, so every load instruction of the wavefront is fully coalesced.
+// Rows are handed to blocks in contiguous per-XCD chunks (see xcd_row_block).
+#include "internal.hpp"
+
+namespace pe {
+
+constexpr int SPMV_WAVES = 4;  // wavefronts (rows) per 256-thread block
+constexpr int NUM_XCD = 8;
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8).
+// Remap so that XCD x receives a contiguous range of row blocks: neighbouring
+// rows of a CI matrix share most of their column targets, so each XCD's L2 then
+// serves a compact slice of X.
+__device__ inline i64 xcd_row_block(i64 b, i64 nblocks) {
+    const i64 per = (nblocks + NUM_XCD - 1) / NUM_XCD;
+    const i64 x = b % NUM_XCD, k = b / NUM_XCD;
+    const i64 r = x * per + k;
+    return r;
+}
+
+template <int NV, typename VT, typename XT>
+__global__ void __launch_bounds__(256) k_spmm(i64 nrow, i64 nblocks, const i64 *__restrict__ rowptr,
+                                              const int32_t *__restrict__ cols, const VT *__restrict__ vals,
+                                              const XT *__restrict__ X, XT *__restrict__ Y) {
+    const int lane = threadIdx.x & 63;
+    const i64 blk = xcd_row_block(blockIdx.x, nblocks);
+    const i64 row = blk * SPMV_WAVES + (threadIdx.x >> 6);
+    if (blk >= nblocks || row >= nrow) return;
+    const i64 beg = rowptr[row], end = rowptr[row + 1];
+    XT acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = (XT)0;
+    i64 p = beg + lane;
+    // two entries in flight per lane per iteration
+    for (; p + 64 < end; p += 128) {
+        const int c0 = cols[p], c1 = cols[p + 64];
+        const XT v0 = (XT)vals[p], v1 = (XT)vals[p + 64];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) acc[j] += v0 * X[(i64)c0 * NV + j] + v1 * X[(i64)c1 * NV + j];
+    }
+    if (p < end) {
+        const int c0 = cols[p];
+        const XT v0 = (XT)vals[p];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) acc[j] += v0 * X[(i64)c0 * NV + j];
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        XT v = acc[j];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        acc[j] = v;
+    }
+    if (lane < NV) {
+        XT out = acc[0];
+#pragma unroll
+        for (int j = 1; j < NV; ++j)
+            if (lane == j) out = acc[j];
+        Y[row * NV + lane] = out;
+    }
+}
+
+// Generic nvec: loops over vector chunks of 8.
+template <typename VT, typename XT>
+__global__ void __launch_bounds__(256) k_spmm_any(i64 nrow, i64 nblocks, int nv, const i64 *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ cols, const VT *__restrict__ vals,
+                                                  const XT *__restrict__ X, XT *__restrict__ Y) {
+    const int lane = threadIdx.x & 63;
+    const i64 blk = xcd_row_block(blockIdx.x, nblocks);
+    const i64 row = blk * SPMV_WAVES + (threadIdx.x >> 6);
+    if (blk >= nblocks || row >= nrow) return;
+    const i64 beg = rowptr[row], end = rowptr[row + 1];
+    for (int j0 = 0; j0 < nv; j0 += 8) {
+        const int cnt = min(8, nv - j0);
+        XT acc[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[j] = (XT)0;
+        for (i64 p = beg + lane; p < end; p += 64) {
+            const int c = cols[p];
+            const XT v = (XT)vals[p];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < cnt) acc[j] += v * X[(i64)c * nv + j0 + j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            XT v = acc[j];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            acc[j] = v;
+        }
+        if (lane < cnt) {
+            XT out = acc[0];
+#pragma unroll
+            for (int j = 1; j < 8; ++j)
+                if (lane == j) out = acc[j];
+            Y[row * nv + j0 + lane] = out;
+        }
+    }
+}
+
+template <typename VT, typename XT>
+static void spmm(const pyci_op *op, const VT *vals, int nvec, const XT *X, XT *Y, hipStream_t s) {
+    const i64 nrow = op->nrow;
+    if (nrow <= 0) return;
+    const i64 nblocks = (nrow + SPMV_WAVES - 1) / SPMV_WAVES;
+    const i64 grid = ((nblocks + NUM_XCD - 1) / NUM_XCD) * NUM_XCD;
+    const i64 *rp = op->rowptr.p;
+    const int32_t *c = op->cols.p;
+    switch (nvec) {
+        case 1: k_spmm<1, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
+        case 2: k_spmm<2, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
+        case 4: k_spmm<4, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
+        case 8: k_spmm<8, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
+        default: k_spmm_any<VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, nvec, rp, c, vals, X, Y); break;
+    }
+    PE_LAUNCH_CHECK();
+}
+
+void sigma_f64(const pyci_op *op, int nvec, const double *X, double *Y, hipStream_t s) {
+    spmm<double, double>(op, op->vals.p, nvec, X, Y, s);
+}
+
+void sigma_f32(const pyci_op *op, int nvec, const float *X, float *Y, hipStream_t s) {
+    spmm<float, float>(op, op->vals32.p, nvec, X, Y, s);
+}
+
+__global__ void k_to_f32(const double *a, float *b, i64 n) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
+void ensure_vals32(pyci_op *op) {
+    if (op->vals32.p && op->vals32.n >= (size_t)op->nnz) return;
+    op->vals32.alloc((size_t)op->nnz);
+    if (op->nnz > 0) {
+        k_to_f32<<<4096, 256, 0, op->ctx->stream>>>(op->vals.p, op->vals32.p, op->nnz);
+        PE_LAUNCH_CHECK();
+    }
+}
+
+}  // namespace pe
+
+namespace pe {
+
+__global__ void k_fill_random(double *x, i64 n, u64 seed) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        x[i] = (double)(mix64(seed + (u64)i) >> 11) * 0x1p-53 - 0.5;
+}
+
+// Times nrep launches of the sigma kernel with HIP events on the context stream.
+// Algorithmic bytes of one launch: every stored entry's column index and value,
+// the row pointers, one read of X and one write of Y.
+void op_bench_sigma(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *avg_ms, double *bytes) {
+    hipStream_t s = op->ctx->stream;
+    const i64 nr = op->nrow, nc = op->ncol;
+    PE_REQUIRE(nvec >= 1 && nrep >= 1, PYCI_EINVAL, "nvec and nrep must be >= 1");
+    const int vb = fp32 ? 4 : 8;
+    if (fp32) ensure_vals32(op);
+    DevBuf<double> x((size_t)(nc * nvec)), y((size_t)(nr * nvec));
+    DevBuf<float> x32, y32;
+    k_fill_random<<<1024, 256, 0, s>>>(x.p, nc * nvec, 12345);
+    PE_LAUNCH_CHECK();
+    if (fp32) {
+        x32.alloc((size_t)(nc * nvec));
+        y32.alloc((size_t)(nr * nvec));
+        k_to_f32<<<1024, 256, 0, s>>>(x.p, x32.p, nc * nvec);
+        PE_LAUNCH_CHECK();
+    }
+    auto launch = [&] {
+        if (fp32)
+            sigma_f32(op, (int)nvec, x32.p, y32.p, s);
+        else
+            sigma_f64(op, (int)nvec, x.p, y.p, s);
+    };
+    launch();  // warm-up
+    PE_HIP(hipEventRecord(op->ctx->ev0, s));
+    for (i64 r = 0; r < nrep; ++r) launch();
+    PE_HIP(hipEventRecord(op->ctx->ev1, s));
+    PE_HIP(hipEventSynchronize(op->ctx->ev1));
+    float ms = 0.f;
+    PE_HIP(hipEventElapsedTime(&ms, op->ctx->ev0, op->ctx->ev1));
+    *avg_ms = ms / (float)nrep;
+    *bytes = (double)op->nnz * (4.0 + vb) + 8.0 * (double)(nr + 1) + (double)vb * (double)nvec * (double)(nc + nr);
+}
+
+}  // namespace pe

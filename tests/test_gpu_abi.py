@@ -1,0 +1,105 @@
+"""GPU parity through the C ABI itself (ctypes on libpyci_engine.so): the calls a
+foreign-language binding would make, checked against the oracle."""
+import ctypes
+import os
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+from pyci_amd.synthetic import config_integrals, seeded_integrals
+from tests.conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+c_i64, c_u64p, c_dp, c_i64p = ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int64), ("sigma_calls", ctypes.c_int64), ("sigma_calls_f32", ctypes.c_int64),
+                ("residual", ctypes.c_double * 64)]
+
+
+@pytest.fixture(scope="module")
+def eng():
+    L = ctypes.CDLL(os.path.join(ROOT, "pyci_amd", "libpyci_engine.so"))
+    L.pyci_last_error.restype = ctypes.c_char_p
+    vp = ctypes.c_void_p
+    L.pyci_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    L.pyci_ctx_destroy.argtypes = [vp]
+    L.pyci_ham_create.argtypes = [vp, c_i64, ctypes.c_double, c_dp, c_dp, ctypes.POINTER(vp)]
+    L.pyci_ham_destroy.argtypes = [vp]
+    L.pyci_wfn_create.argtypes = [vp, ctypes.c_int, c_i64, c_i64, c_i64, c_i64, c_u64p, ctypes.POINTER(vp)]
+    L.pyci_wfn_destroy.argtypes = [vp]
+    L.pyci_wfn_index.argtypes = [vp, c_i64, c_u64p, c_i64p]
+    L.pyci_fill_all_dets.argtypes = [vp, ctypes.c_int, c_i64, c_i64, c_i64, c_u64p]
+    L.pyci_op_create.argtypes = [vp, vp, c_i64, c_i64, ctypes.c_int, ctypes.POINTER(vp)]
+    L.pyci_op_destroy.argtypes = [vp]
+    L.pyci_op_info.argtypes = [vp, c_i64p, c_i64p, c_i64p, c_i64p, ctypes.POINTER(ctypes.c_int), c_dp]
+    L.pyci_op_export.argtypes = [vp, c_dp, c_i64p, c_i64p]
+    L.pyci_op_matvec.argtypes = [vp, c_i64, c_dp, c_dp]
+    L.pyci_op_solve.argtypes = [vp, c_i64, c_dp, c_i64, c_i64, ctypes.c_double, ctypes.c_int, c_dp, c_dp,
+                                ctypes.POINTER(Stats)]
+    L.pyci_op_bench_sigma.argtypes = [vp, c_i64, c_i64, ctypes.c_int, ctypes.POINTER(ctypes.c_float), c_dp]
+    ctx = vp()
+    assert L.pyci_ctx_create(0, ctypes.byref(ctx)) == 0, L.pyci_last_error()
+    yield L, ctx
+    L.pyci_ctx_destroy(ctx)
+
+
+def p(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def test_abi_fci8_build_solve(eng, oracle):
+    L, ctx = eng
+    _, one, two = config_integrals("fci8")
+    dets = np.zeros((4900, 2), dtype=np.uint64)
+    assert L.pyci_fill_all_dets(ctx, 1, 8, 4, 4, p(dets, ctypes.c_uint64)) == 0
+    npt.assert_array_equal(dets, oracle.all_dets("fullci", 8, 4, 4))
+    ham, wfn, op = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    assert L.pyci_ham_create(ctx, 8, 0.0, p(one, ctypes.c_double), p(two, ctypes.c_double), ctypes.byref(ham)) == 0
+    assert L.pyci_wfn_create(ctx, 1, 8, 4, 4, 4900, p(dets, ctypes.c_uint64), ctypes.byref(wfn)) == 0
+    assert L.pyci_op_create(ham, wfn, -1, -1, 1, ctypes.byref(op)) == 0, L.pyci_last_error()
+    nr, nc, sz, nnz = (ctypes.c_int64() for _ in range(4))
+    sym, ec = ctypes.c_int(), ctypes.c_double()
+    assert L.pyci_op_info(op, ctypes.byref(nr), ctypes.byref(nc), ctypes.byref(sz), ctypes.byref(nnz),
+                          ctypes.byref(sym), ctypes.byref(ec)) == 0
+    assert (nr.value, nc.value, sz.value, nnz.value, sym.value) == (4900, 4900, 886900, 1768900, 1)
+    data, idx, ptr = np.zeros(sz.value), np.zeros(sz.value, np.int64), np.zeros(4901, np.int64)
+    assert L.pyci_op_export(op, p(data, ctypes.c_double), p(idx, ctypes.c_int64), p(ptr, ctypes.c_int64)) == 0
+    ref = oracle.csr_build("fullci", 8, 4, 4, dets, one, two)
+    npt.assert_array_equal(ptr, ref.indptr)
+    npt.assert_array_equal(idx, ref.indices)
+    npt.assert_array_equal(data, ref.data)
+    x = np.random.default_rng(1).normal(size=(2, 4900))
+    y = np.zeros((2, 4900))
+    assert L.pyci_op_matvec(op, 2, p(x, ctypes.c_double), p(y, ctypes.c_double)) == 0
+    npt.assert_allclose(y, (ref.scipy() @ x.T).T, rtol=1e-11, atol=1e-11)
+    ev, vec, st = np.zeros(2), np.zeros((2, 4900)), Stats()
+    assert L.pyci_op_solve(op, 2, None, -1, -1, 1e-12, 0, p(ev, ctypes.c_double), p(vec, ctypes.c_double),
+                           ctypes.byref(st)) == 0
+    w = np.linalg.eigvalsh(ref.dense())
+    npt.assert_allclose(ev, w[:2][::-1], atol=1e-10)
+    ms, nb = ctypes.c_float(), ctypes.c_double()
+    assert L.pyci_op_bench_sigma(op, 1, 3, 0, ctypes.byref(ms), ctypes.byref(nb)) == 0
+    assert ms.value > 0 and nb.value == 1768900 * 12 + 8 * 4901 + 8 * 2 * 4900
+    q = np.ascontiguousarray(dets[[5, 77, 4899]])
+    out = np.zeros(3, np.int64)
+    assert L.pyci_wfn_index(wfn, 3, p(q, ctypes.c_uint64), p(out, ctypes.c_int64)) == 0
+    assert out.tolist() == [5, 77, 4899]
+    assert L.pyci_op_solve(op, 4900, None, -1, -1, 1e-12, 0, p(ev, ctypes.c_double), p(vec, ctypes.c_double),
+                           None) == 1  # n >= nrow -> PYCI_EINVAL
+    L.pyci_op_destroy(op)
+    L.pyci_wfn_destroy(wfn)
+    L.pyci_ham_destroy(ham)
+
+
+def test_abi_rejects_bad_shapes(eng):
+    L, ctx = eng
+    one, two = np.zeros((70, 70)), np.zeros(1)
+    ham = ctypes.c_void_p()
+    assert L.pyci_ham_create(ctx, 70, 0.0, p(one, ctypes.c_double), p(two, ctypes.c_double), ctypes.byref(ham)) == 6
+    dets = np.zeros(2, dtype=np.uint64)
+    wfn = ctypes.c_void_p()
+    assert L.pyci_wfn_create(ctx, 1, 4, 1, 2, 1, p(dets, ctypes.c_uint64), ctypes.byref(wfn)) == 1

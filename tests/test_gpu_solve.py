@@ -1,0 +1,90 @@
+"""GPU Davidson: known-answer energies held by the reference, eigenpairs versus
+the oracle, ordering conventions and error behaviour (sparseop.cpp:114-146)."""
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+import pyci_amd as pyci
+from pyci_amd.synthetic import config_integrals, seeded_integrals
+from tests.conftest import datafile
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("fn,wtype,occs,energy", [
+    ("li2_ccpvdz", pyci.doci_wfn, (3, 3), -14.878455349),
+    ("be_ccpvdz", pyci.doci_wfn, (2, 2), -14.600556994),
+    ("be_ccpvdz", pyci.fullci_wfn, (2, 2), -14.617409507),
+    ("h2o_ccpvdz", pyci.doci_wfn, (5, 5), -75.634588422),
+])
+def test_solve_sparse(fn, wtype, occs, energy):
+    """test_routines.py:37-54 (atol 1e-9)."""
+    ham = pyci.secondquant_op(datafile(fn + ".fcidump"))
+    wfn = wtype(ham.nbasis, *occs)
+    wfn.add_all_dets()
+    op = pyci.sparse_op(ham, wfn)
+    es, cs = op.solve(n=1, ncv=30, tol=1.0e-6)
+    npt.assert_allclose(es[0], energy, rtol=0.0, atol=1.0e-9)
+    assert cs.shape == (1, len(wfn))
+    npt.assert_allclose(np.linalg.norm(cs[0]), 1.0, atol=1e-12)
+
+
+def test_fci8_against_oracle(oracle):
+    _, one, two = config_integrals("fci8")
+    w = pyci.fullci_wfn(8, 4, 4)
+    w.add_all_dets()
+    op = pyci.sparse_op(pyci.hamiltonian(0.0, one, two), w)
+    csr = oracle.csr_build("fullci", 8, 4, 4, oracle.all_dets("fullci", 8, 4, 4), one, two)
+    ref = np.linalg.eigvalsh(csr.dense())
+    es, cs = op.solve(n=1)
+    assert abs(es[0] - ref[0]) < 1e-10
+    es3, cs3 = op.solve(n=3)
+    npt.assert_allclose(es3, ref[:3][::-1], atol=1e-10)  # descending for n > 1
+    H = csr.scipy()
+    for k in range(3):
+        r = H @ cs3[k] - es3[k] * cs3[k]
+        assert np.linalg.norm(r) < 1e-9
+    esm, _ = op.solve(n=1, precision="mixed")
+    assert abs(esm[0] - ref[0]) < 1e-10
+
+
+def test_h4_doci_spectrum_order():
+    """H4/STO-3G DOCI: five of six roots, descending (test_odometer.py:38-79)."""
+    ham = pyci.hamiltonian(datafile("h4_sto3g.fcidump"))
+    w = pyci.doci_wfn(ham.nbasis, 2, 2)
+    w.add_all_dets()
+    op = pyci.sparse_op(ham, w)
+    es, cs = op.solve(n=5, tol=1e-9)
+    npt.assert_allclose(es, [0.10356075, -0.26165295, -0.8372566, -1.3971786, -2.13398665], atol=1e-7)
+    cost = -es
+    assert cost[0] + cost[1] < 0.16 and cost[0] < 0.15 + 0.01
+
+
+def test_h4_fullci_ground():
+    ham = pyci.hamiltonian(datafile("h4_sto3g.fcidump"))
+    w = pyci.fullci_wfn(ham.nbasis, 2, 2)
+    w.add_all_dets()
+    es, _ = pyci.sparse_op(ham, w).solve(n=len(w) - 1, tol=1e-9)
+    assert -1.175 <= es[-1] < -1.174
+
+
+def test_solve_errors_and_warm_start():
+    _, one, two = seeded_integrals(5, 6, 50)
+    ham = pyci.hamiltonian(0.0, one, two)
+    w = pyci.fullci_wfn(5, 2, 2)
+    w.add_all_dets()
+    op = pyci.sparse_op(ham, w)
+    with pytest.raises(ValueError):
+        op.solve(n=len(w))
+    with pytest.raises(RuntimeError):
+        op.solve(n=1, maxiter=1)
+    rect = pyci.sparse_op(ham, w, len(w) - 3, symmetric=False)
+    with pytest.raises(TypeError):
+        rect.solve(n=1)
+    e0, c0 = op.solve(n=1)
+    e1, c1 = op.solve(n=1, c0=c0[0])
+    assert abs(e0[0] - e1[0]) < 1e-11
+    one_det = pyci.fullci_wfn(5, 2, 2)
+    one_det.add_hartreefock_det()
+    e, c = pyci.sparse_op(ham, one_det).solve()
+    assert c.tolist() == [[1.0]] and e[0] == pyci.sparse_op(ham, one_det).get_element(0, 0)
