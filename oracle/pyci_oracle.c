@@ -295,7 +295,8 @@ i64 oracle_add_excited_dets(int kind, i64 nbasis, i64 nup, i64 ndn, i64 ndet, co
 /* ------------------------------------------------------------------------ */
 
 typedef struct {
-    i64 nrow, ncol, nnz, cap;
+    i64 row0;               /* global index of the first stored row */
+    i64 nrow, ncol, nnz, cap; /* nrow = number of stored rows */
     double *data;
     i64 *indices;
     i64 *indptr;
@@ -534,7 +535,8 @@ oracle_csr *oracle_csr_build(int kind, i64 nbasis, i64 nup, i64 ndn, i64 ndet, c
     hamv H;
     ham_init(&H, kind, nbasis, nup, ndn, one_mo, two_mo);
     oracle_csr *c = (oracle_csr *)calloc(1, sizeof(oracle_csr));
-    c->nrow = nrow;
+    c->row0 = row0;
+    c->nrow = nrow - row0;
     c->ncol = ncol;
     c->indptr = (i64 *)malloc(sizeof(i64) * (nrow - row0 + 1));
     c->indptr[0] = 0;
@@ -570,16 +572,19 @@ void oracle_csr_free(oracle_csr *c) {
 
 /* perform_op / perform_op_symm (sparseop.cpp:96-112): y = A x, or with a
  * lower-stored symmetric matrix y = (L + L^T - D) x. ecore is not added. */
+/* Rows are global rows row0 .. row0+nrow-1; x has ncol entries and y must hold
+ * max(row0+nrow, ncol) entries (the transposed half writes y[col]). */
 void oracle_csr_matvec(const oracle_csr *c, int symmetric, const double *x, double *y) {
-    for (i64 i = 0; i < c->nrow; ++i) y[i] = 0.0;
+    for (i64 i = 0; i < c->nrow; ++i) y[c->row0 + i] = 0.0;
     for (i64 i = 0; i < c->nrow; ++i) {
+        const i64 gi = c->row0 + i;
         double acc = 0.0;
         for (i64 p = c->indptr[i]; p < c->indptr[i + 1]; ++p) {
             i64 j = c->indices[p];
             acc += c->data[p] * x[j];
-            if (symmetric && j != i) y[j] += c->data[p] * x[i];
+            if (symmetric && j != gi) y[j] += c->data[p] * x[gi];
         }
-        y[i] += acc;
+        y[gi] += acc;
     }
 }
 

@@ -334,9 +334,9 @@ PYBIND11_MODULE(_pyci, m) {
              py::arg("maxiter") = -1, py::arg("tol") = 1.0e-12, py::arg("precision") = "f64")
         .def("reserve", [](SparseOp &, i64) {}, py::arg("n"))
         .def("squeeze", [](SparseOp &) {})
-        .def("data", [](const SparseOp &o) { return o.export_csr()[0]; })
-        .def("indices", [](const SparseOp &o) { return o.export_csr()[1]; })
-        .def("indptr", [](const SparseOp &o) { return o.export_csr()[2]; })
+        .def("data", [](const SparseOp &o) { return py::object(o.export_csr()[0]); })
+        .def("indices", [](const SparseOp &o) { return py::object(o.export_csr()[1]); })
+        .def("indptr", [](const SparseOp &o) { return py::object(o.export_csr()[2]); })
         .def("export", &SparseOp::export_csr)
         .def("diagonal", &SparseOp::diagonal);
 

@@ -453,8 +453,10 @@ int pyci_op_solve(pyci_op *op, int64_t n, const double *c0, int64_t ncv, int64_t
         set_device(op->ctx);
         if (stats) std::memset(stats, 0, sizeof(*stats));
         if (nr == 1) {
+            // the diagonal is written by the build kernels on the context stream
             double d = 0.0;
-            PE_HIP(hipMemcpy(&d, op->diag.p, sizeof(double), hipMemcpyDeviceToHost));
+            PE_HIP(hipMemcpyAsync(&d, op->diag.p, sizeof(double), hipMemcpyDeviceToHost, op->ctx->stream));
+            PE_HIP(hipStreamSynchronize(op->ctx->stream));
             evals[0] = d + op->ecore;
             evecs[0] = 1.0;
             return;

@@ -32,7 +32,15 @@ void set_error(const std::string &msg);
             throw ::pe::EngineError(PYCI_EHIP, std::string(#x " failed: ") + hipGetErrorString(e_)); \
     } while (0)
 
-#define PE_LAUNCH_CHECK() PE_HIP(hipGetLastError())
+#define PE_STR2(x) #x
+#define PE_STR(x) PE_STR2(x)
+#define PE_LAUNCH_CHECK()                                                                                \
+    do {                                                                                                 \
+        hipError_t e_ = hipGetLastError();                                                               \
+        if (e_ != hipSuccess)                                                                            \
+            throw ::pe::EngineError(PYCI_EHIP, std::string("kernel launch at " __FILE__ ":" PE_STR(__LINE__) \
+                                                           " failed: ") + hipGetErrorString(e_));         \
+    } while (0)
 
 #define PE_REQUIRE(cond, code, msg)                     \
     do {                                                \
@@ -71,7 +79,11 @@ struct DevBuf {
         n = count;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            // a failed free must not linger as the thread's last HIP error, where it
+            // would be reported against an unrelated later launch
+            if (hipFree(p) != hipSuccess) (void)hipGetLastError();
+        }
         p = nullptr;
         n = 0;
     }

@@ -60,12 +60,20 @@ def test_h4_doci_spectrum_order():
     assert cost[0] + cost[1] < 0.16 and cost[0] < 0.15 + 0.01
 
 
-def test_h4_fullci_ground():
+def test_h4_fullci_roots(oracle):
+    """Sixteen lowest H4/STO-3G FullCI roots against the dense oracle spectrum
+    (lowest -2.16638745 Eh with ecore); more than 16 roots per call is rejected."""
     ham = pyci.hamiltonian(datafile("h4_sto3g.fcidump"))
     w = pyci.fullci_wfn(ham.nbasis, 2, 2)
     w.add_all_dets()
-    es, _ = pyci.sparse_op(ham, w).solve(n=len(w) - 1, tol=1e-9)
-    assert -1.175 <= es[-1] < -1.174
+    op = pyci.sparse_op(ham, w)
+    es, _ = op.solve(n=16, tol=1e-9)
+    ref = np.linalg.eigvalsh(oracle.csr_build("fullci", 4, 2, 2, oracle.all_dets("fullci", 4, 2, 2), ham.one_mo,
+                                              ham.two_mo).dense()) + ham.ecore
+    npt.assert_allclose(es, ref[:16][::-1], atol=1e-9)
+    assert abs(es[-1] - (-2.16638745)) < 1e-7
+    with pytest.raises(ValueError):
+        op.solve(n=17)
 
 
 def test_solve_errors_and_warm_start():

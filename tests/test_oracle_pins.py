@@ -230,3 +230,16 @@ def test_hci_matches_brute_force_criterion(oracle):
                 if j not in idx and abs(H[j, i]) > eps / abs(c[col]):
                     want.add(tuple(alld[j]))
         assert got == want and len(new) == len(got)
+
+
+def test_partial_row_range(oracle):
+    """Rows row0..nrow-1 only: the row pointers have (nrow - row0 + 1) entries and
+    the rows equal the corresponding rows of the full build."""
+    _, one, two = seeded_integrals(6, 6, 41)
+    dets = oracle.all_dets("fullci", 6, 2, 2)
+    full = oracle.csr_build("fullci", 6, 2, 2, dets, one, two, symmetric=False)
+    part = oracle.csr_build("fullci", 6, 2, 2, dets, one, two, row0=100, nrow=150, symmetric=False)
+    assert part.indptr.size == 51 and part.shape == (50, len(dets))
+    a, b = full.indptr[100], full.indptr[150]
+    np.testing.assert_array_equal(part.indices, full.indices[a:b])
+    np.testing.assert_array_equal(part.indptr, full.indptr[100:151] - a)
