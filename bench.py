@@ -81,6 +81,36 @@ def dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def measured_traffic(name):
+    """HBM bytes per sigma launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (profiles/sigma_<workload>_hbm_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", f"sigma_{name}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("| traffic = read + write |"):
+            return float(line.split("|")[2].split()[0]) * 1e9
+    return None
+
+
+def device_copy(nbytes):
+    """One device-to-device copy of nbytes through the HIP runtime (a plain copy
+    kernel of known size, used to calibrate the HBM byte counters)."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    t0 = time.perf_counter()
+    assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    t = time.perf_counter() - t0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return t
+
+
 def cpu_baseline(wl, one, two, dets, sigma_calls, budget_s):
     """Oracle (serial C restatement of the reference path) on a bounded row sample."""
     from oracle import oracle as O
@@ -136,6 +166,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sigma-only", action="store_true",
+                    help="build the operator, run a few sigma launches and a device copy of known size "
+                         "(for hardware-counter runs), print their timings and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -181,6 +214,16 @@ def main():
         ok(L, L.pyci_op_solve(op, nroot, None, -1, -1, tol, p, dp(ev), dp(vec), ctypes.byref(st)), "op_solve")
         return ev, st
 
+    if args.sigma_only:
+        op = build()
+        nrow, ncol, size_ref, nnz = info(op)
+        ms, nbytes = ctypes.c_float(), ctypes.c_double()
+        ok(L, L.pyci_op_bench_sigma(op, 1, 4, 0, ctypes.byref(ms), ctypes.byref(nbytes)), "bench_sigma")
+        copy_bytes = 4 << 30
+        t_copy = device_copy(copy_bytes)
+        print(json.dumps({"sigma_avg_ms": ms.value, "sigma_bytes": nbytes.value, "nnz": nnz,
+                          "copy_bytes": copy_bytes, "copy_ms": t_copy * 1e3}), flush=True)
+        return
     # FP64 reference energy (untimed, tight tolerance)
     op = build()
     nrow, ncol, size_ref, nnz = info(op)
@@ -231,8 +274,9 @@ def main():
     max_de = max(r[2] for r in res)
     sigma_calls = int(np.mean([r[4] for r in res]))
     achieved = sig["bytes"] / (sig["avg_ms"] * 1e-3) / 1e9
+    traffic = measured_traffic(wl["name"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "kernel": "k_spmm<1,double,double> (sigma, FP64 values)", "avg_ms": round(sig["avg_ms"], 4),
                 "bytes_per_launch": sig["bytes"]}
     out = {
