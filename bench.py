@@ -277,7 +277,7 @@ def main():
     traffic = measured_traffic(wl["name"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                "kernel": "k_spmm<1,double,double> (sigma, FP64 values)", "avg_ms": round(sig["avg_ms"], 4),
+                "kernel": "k_spmv<4,double,double> (sigma, FP64 values)", "avg_ms": round(sig["avg_ms"], 4),
                 "bytes_per_launch": sig["bytes"]}
     out = {
         "metric": "time-to-|dE|<1e-8 Eh vs FP64 oracle; sigma HBM GB/s; dets/s built",

@@ -66,12 +66,16 @@ __global__ void __launch_bounds__(DT) k_gram(i64 nd, const double *A, i64 lda, i
     }
 }
 
-__global__ void k_reduce_partials(const double *partial, int nblk, int count, double *out) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
+// out[t] = sum over b of partial[b][t]; one 256-thread block per output element.
+__global__ void __launch_bounds__(256) k_reduce_partials(const double *partial, int nblk, int count, double *out) {
+    const int t = blockIdx.x;
     double v = 0.0;
-    for (int b = 0; b < nblk; ++b) v += partial[(i64)b * count + t];
-    out[t] = v;
+    for (int b = threadIdx.x; b < nblk; b += 256) v += partial[(i64)b * count + t];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[t] = red[0] + red[1] + red[2] + red[3];
 }
 
 // T[:, j] += alpha * V[:, 0:m] C[:, j]   (C is m x k column-major on the device)
@@ -241,7 +245,7 @@ struct Davidson {
         k_gram<<<grid, DT, 0, s>>>(nd, A, nd, ma, B, nd, mb, partial.p);
         PE_LAUNCH_CHECK();
         int cnt = ma * mb;
-        k_reduce_partials<<<(cnt + 255) / 256, 256, 0, s>>>(partial.p, nblk, cnt, dst);
+        k_reduce_partials<<<(unsigned)cnt, 256, 0, s>>>(partial.p, nblk, cnt, dst);
         PE_LAUNCH_CHECK();
         // dst is [i][j] row-major (i < ma); as column-major m x k with m = ma it is
         // transposed, so callers that need C (ma x mb col-major) use gram_cm.
@@ -249,6 +253,11 @@ struct Davidson {
 
     // C (ma x mb column-major, on device) = A^T B
     void gram_cm(const double *A, int ma, const double *B, int mb, double *C) {
+        if (mb == 1) {
+            // [ma][1] row-major is already ma x 1 column-major: no host round trip
+            gram(A, ma, B, mb, C);
+            return;
+        }
         gram(A, ma, B, mb, small.p + (size_t)mmax * mmax);
         // transpose row-major [ma][mb] to column-major via host (tiny)
         std::vector<double> h((size_t)ma * mb), t((size_t)ma * mb);

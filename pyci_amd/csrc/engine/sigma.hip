@@ -67,6 +67,39 @@ __global__ void __launch_bounds__(256) k_spmm(i64 nrow, i64 nblocks, const i64 *
     }
 }
 
+// Single vector: every lane issues U column/value loads (non-temporal: the CSR is
+// streamed once per sigma and must not evict X from L2) before their U dependent
+// X gathers, so each wavefront keeps U*64 entries in flight.
+template <int U, typename VT, typename XT>
+__global__ void __launch_bounds__(256) k_spmv(i64 nrow, i64 nblocks, const i64 *__restrict__ rowptr,
+                                              const int32_t *__restrict__ cols, const VT *__restrict__ vals,
+                                              const XT *__restrict__ X, XT *__restrict__ Y) {
+    const int lane = threadIdx.x & 63;
+    const i64 blk = xcd_row_block(blockIdx.x, nblocks);
+    const i64 row = blk * SPMV_WAVES + (threadIdx.x >> 6);
+    if (blk >= nblocks || row >= nrow) return;
+    const i64 beg = rowptr[row], end = rowptr[row + 1];
+    XT acc = (XT)0;
+    i64 p = beg + lane;
+    for (; p + (U - 1) * 64 < end; p += U * 64) {
+        int c[U];
+        VT v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            c[u] = __builtin_nontemporal_load(&cols[p + u * 64]);
+            v[u] = __builtin_nontemporal_load(&vals[p + u * 64]);
+        }
+        XT x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = X[c[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc += (XT)v[u] * x[u];
+    }
+    for (; p < end; p += 64) acc += (XT)__builtin_nontemporal_load(&vals[p]) * X[__builtin_nontemporal_load(&cols[p])];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) Y[row] = acc;
+}
+
 // Generic nvec: loops over vector chunks of 8.
 template <typename VT, typename XT>
 __global__ void __launch_bounds__(256) k_spmm_any(i64 nrow, i64 nblocks, int nv, const i64 *__restrict__ rowptr,
@@ -114,7 +147,7 @@ static void spmm(const pyci_op *op, const VT *vals, int nvec, const XT *X, XT *Y
     const i64 *rp = op->rowptr.p;
     const int32_t *c = op->cols.p;
     switch (nvec) {
-        case 1: k_spmm<1, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
+        case 1: k_spmv<4, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
         case 2: k_spmm<2, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
         case 4: k_spmm<4, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
         case 8: k_spmm<8, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
