@@ -44,6 +44,11 @@ extern "C" {
 #define PYCI_PREC_F64 0   /* FP64 sigma throughout */
 #define PYCI_PREC_MIXED 1 /* FP32-value sigma, then FP64 residual-correction pass */
 
+/* Operator storage */
+#define PYCI_STORAGE_CSR 0    /* CSR in HBM (the reference's SparseOp) */
+#define PYCI_STORAGE_DIRECT 1 /* matrix-free FullCI sigma; the CSR is materialised on demand */
+#define PYCI_STORAGE_AUTO 2   /* DIRECT where it applies, else CSR */
+
 typedef struct pyci_ctx pyci_ctx; /* one device + one HIP stream */
 typedef struct pyci_ham pyci_ham; /* integrals resident in HBM */
 typedef struct pyci_wfn pyci_wfn; /* determinant set + device index */
@@ -107,6 +112,15 @@ int pyci_fill_all_dets(pyci_ctx *ctx, int kind, int64_t nbasis, int64_t nocc_up,
  * 0..nrow-1 with column < ncol, rows sorted by column; with symmetric != 0 the
  * reference-format export is the lower triangle plus diagonal. */
 int pyci_op_create(pyci_ham *ham, pyci_wfn *wfn, int64_t nrow, int64_t ncol, int symmetric, pyci_op **op);
+/* Same with a storage choice (PYCI_STORAGE_*). DIRECT (matrix-free sigma)
+ * applies to the full FullCI space in colex order (add_all_dets), square and
+ * symmetric, nbasis <= 15, both spins occupied; it stores string tables and
+ * integrals instead of the CSR, which export/get_element build on first use.
+ * AUTO picks DIRECT where it applies. Same operator semantics as SparseOp. */
+int pyci_op_create_ex(pyci_ham *ham, pyci_wfn *wfn, int64_t nrow, int64_t ncol, int symmetric, int storage,
+                      pyci_op **op);
+/* Storage actually used by an operator (PYCI_STORAGE_CSR or PYCI_STORAGE_DIRECT). */
+int pyci_op_storage(const pyci_op *op, int *storage);
 /* update(ham, wfn) (sparseop.cpp:175-178): extends the operator to the wave
  * function's current ndet rows and columns. */
 int pyci_op_update(pyci_op *op, pyci_ham *ham, pyci_wfn *wfn);
@@ -154,6 +168,48 @@ int pyci_add_hci(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs, double eps,
  *           <a+_p a_q>, <a+_p a+_q a_s a_r>; the reference routine at
  *           rdm.cpp:532-632 has index errors, so it is not reproduced). */
 int pyci_compute_rdms(pyci_wfn *wfn, const double *coeffs, double *rdm1, double *rdm2);
+
+/* ------------------------------------------------------------------------
+ * Learned selector. No counterpart in the reference: it proposes which
+ * heat-bath candidates add_hci (hci.cpp:252-279) appends.
+ *
+ * Network: occupation bits (nfeat <= 64: the determinant's bits; for FullCI
+ * alpha bits 0..nbasis-1 then beta bits nbasis..2*nbasis-1) -> 256 -> 256 -> 1,
+ * ReLU, predicting log10|c|. BF16 operands with FP32 accumulation on MFMA;
+ * FP32 master parameters, Adam. Flat FP32 parameter layout (PYCI_MLP_NPARAM):
+ *   W1[64][256] (input x hidden; rows >= nfeat are zero), b1[256],
+ *   W2[256][256], b2[256], w3[256], b3.
+ * Forward: h1 = bf16(relu(bf16(x) W1b + b1)), h2 = bf16(relu(h1 W2b + b2)),
+ * y = h2 . w3 + b3, with W1b, W2b the bf16-rounded weights.
+ * ------------------------------------------------------------------------ */
+#define PYCI_MLP_HIDDEN 256
+#define PYCI_MLP_NPARAM (64 * 256 + 256 + 256 * 256 + 256 + 256 + 1)
+typedef struct pyci_mlp pyci_mlp;
+
+/* Creates a network with seeded He-uniform weights (zero biases). */
+int pyci_mlp_create(pyci_ctx *ctx, int64_t nfeat, uint64_t seed, pyci_mlp **out);
+int pyci_mlp_destroy(pyci_mlp *m);
+int pyci_mlp_get_params(pyci_mlp *m, float *params);       /* PYCI_MLP_NPARAM floats */
+int pyci_mlp_set_params(pyci_mlp *m, const float *params); /* resets the Adam state */
+/* y[n] = network(dets) for host determinants (nkey words each: 2 for FullCI). */
+int pyci_mlp_predict(pyci_mlp *m, int kind, int64_t nbasis, int64_t n, const uint64_t *dets, float *y);
+/* Mean-squared-error loss over n samples and its gradient (PYCI_MLP_NPARAM floats). */
+int pyci_mlp_loss_grad(pyci_mlp *m, int kind, int64_t nbasis, int64_t n, const uint64_t *dets,
+                       const float *target, float *loss, float *grad);
+/* Trains on the wave function's determinants with target log10|coeffs| (host
+ * coeffs, ndet float64; determinants with |c| < cmin are left out): `steps`
+ * Adam updates on minibatches of `batch` samples drawn with a counter-based
+ * generator from `seed`. Returns the loss of the last minibatch. */
+int pyci_mlp_train(pyci_mlp *m, const pyci_wfn *wfn, const double *coeffs, double cmin, int64_t steps,
+                   int64_t batch, double lr, uint64_t seed, float *last_loss);
+
+/* add_hci with the learned selector: the heat-bath pool is taken at eps/pool
+ * (pool >= 1); `nadd` < 0 appends as many determinants as plain add_hci(eps)
+ * would, otherwise min(nadd, pool size); candidates are ranked by the network
+ * (score_mode 0) or by network + log10 of the heat-bath score (1); ties keep
+ * the first-occurrence order. nadded may be NULL. */
+int pyci_add_hci_mlp(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs, double eps, pyci_mlp *m, double pool,
+                     int64_t nadd, int score_mode, int64_t *nadded);
 
 #ifdef __cplusplus
 }

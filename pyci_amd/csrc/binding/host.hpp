@@ -172,8 +172,9 @@ struct SparseOp {
     double ecore = 0.0;
     bool symmetric = true;
     py::tuple shape;
+    std::string storage = "csr";
 
-    SparseOp(SQuantOp &ham, Wfn &wfn, i64 nrow, i64 ncol, bool symm);
+    SparseOp(SQuantOp &ham, Wfn &wfn, i64 nrow, i64 ncol, bool symm, const std::string &storage);
     void refresh();
     void update(SQuantOp &ham, Wfn &wfn);
     Array<double> matvec(Array<double> x) const;

@@ -5,11 +5,16 @@
 
 namespace ph {
 
-SparseOp::SparseOp(SQuantOp &ham, Wfn &wfn, i64 rows, i64 cols, bool symm) {
+SparseOp::SparseOp(SQuantOp &ham, Wfn &wfn, i64 rows, i64 cols, bool symm, const std::string &storage_) {
     if (ham.nbasis != wfn.nbasis)
         throw py::value_error("hamiltonian and wave function have different nbasis");
+    int st;
+    if (storage_ == "csr") st = PYCI_STORAGE_CSR;
+    else if (storage_ == "direct") st = PYCI_STORAGE_DIRECT;
+    else if (storage_ == "auto") st = PYCI_STORAGE_AUTO;
+    else throw py::value_error("storage must be 'auto', 'csr' or 'direct'");
     pyci_op *o = nullptr;
-    check(pyci_op_create(ham.device(), wfn.device(), rows, cols, symm ? 1 : 0, &o));
+    check(pyci_op_create_ex(ham.device(), wfn.device(), rows, cols, symm ? 1 : 0, st, &o));
     op = std::shared_ptr<pyci_op>(o, pyci_op_destroy);
     refresh();
 }
@@ -25,6 +30,9 @@ void SparseOp::refresh() {
     symmetric = sym != 0;
     ecore = ec;
     shape = py::make_tuple(nrow, ncol);
+    int st = PYCI_STORAGE_CSR;
+    check(pyci_op_storage(op.get(), &st));
+    storage = st == PYCI_STORAGE_DIRECT ? "direct" : "csr";
 }
 
 void SparseOp::update(SQuantOp &ham, Wfn &wfn) {

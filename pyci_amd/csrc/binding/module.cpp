@@ -137,17 +137,69 @@ static void add_dets_from(Wfn &dst, const Wfn &src) {
     for (i64 i = 0; i < src.ndet; ++i) dst.add_det(src.det_ptr(i));
 }
 
+// Learned selector (no counterpart in the reference): the network of
+// pyci_mlp_* over the determinants' occupation bits.
+struct MLPSelector {
+    std::shared_ptr<pyci_mlp> net;
+    i64 nfeat = 0;
+    MLPSelector(i64 nf, u64 seed) : nfeat(nf) {
+        pyci_mlp *m = nullptr;
+        check(pyci_mlp_create(context(), nf, seed, &m));
+        net.reset(m, [](pyci_mlp *p) { pyci_mlp_destroy(p); });
+    }
+    Array<float> get_params() const {
+        Array<float> p((i64)PYCI_MLP_NPARAM);
+        check(pyci_mlp_get_params(net.get(), p.mutable_data()));
+        return p;
+    }
+    void set_params(Array<float> p) {
+        if (p.size() != PYCI_MLP_NPARAM) throw py::value_error("expected PYCI_MLP_NPARAM parameters");
+        check(pyci_mlp_set_params(net.get(), p.data()));
+    }
+    Array<float> predict(const Wfn &w) const {
+        Array<float> y(w.ndet);
+        if (w.ndet == 0) return y;
+        py::gil_scoped_release nogil;
+        check(pyci_mlp_predict(net.get(), w.kind, w.nbasis, w.ndet, w.dets.data(), y.mutable_data()));
+        return y;
+    }
+    py::tuple loss_grad(const Wfn &w, Array<float> target) const {
+        if ((i64)target.size() != w.ndet) throw py::value_error("target must have one entry per determinant");
+        Array<float> g((i64)PYCI_MLP_NPARAM);
+        float loss = 0.0f;
+        {
+            py::gil_scoped_release nogil;
+            check(pyci_mlp_loss_grad(net.get(), w.kind, w.nbasis, w.ndet, w.dets.data(), target.data(), &loss,
+                                     g.mutable_data()));
+        }
+        return py::make_tuple(loss, g);
+    }
+    float train(Wfn &w, Array<double> coeffs, i64 steps, i64 batch, double lr, double cmin, u64 seed) {
+        if ((i64)coeffs.size() != w.ndet) throw py::value_error("coeffs must have one entry per determinant");
+        pyci_wfn *dw = w.device();
+        float loss = 0.0f;
+        py::gil_scoped_release nogil;
+        check(pyci_mlp_train(net.get(), dw, coeffs.data(), cmin, steps, batch, lr, seed, &loss));
+        return loss;
+    }
+};
+
 template <class W>
-static i64 add_hci(SQuantOp &ham, W &wfn, Array<double> coeffs, double eps, i64) {
+static i64 add_hci(SQuantOp &ham, W &wfn, Array<double> coeffs, double eps, i64, py::object selector, double pool,
+                   i64 nadd, int score_mode) {
     if ((i64)coeffs.size() != wfn.ndet) throw py::value_error("coeffs must have one entry per determinant");
     if (ham.nbasis != wfn.nbasis) throw py::value_error("hamiltonian and wave function have different nbasis");
+    MLPSelector *sel = selector.is_none() ? nullptr : selector.cast<MLPSelector *>();
     pyci_wfn *dw = wfn.device();
     pyci_ham *dh = ham.device();
     const i64 old = wfn.ndet;
     int64_t added = 0;
     {
         py::gil_scoped_release nogil;
-        check(pyci_add_hci(dh, dw, coeffs.data(), eps, &added));
+        if (sel)
+            check(pyci_add_hci_mlp(dh, dw, coeffs.data(), eps, sel->net.get(), pool, nadd, score_mode, &added));
+        else
+            check(pyci_add_hci(dh, dw, coeffs.data(), eps, &added));
     }
     wfn.adopt_device_appends(old);
     return added;
@@ -315,13 +367,23 @@ PYBIND11_MODULE(_pyci, m) {
         .def_readonly("symmetric", &SparseOp::symmetric)
         .def_readonly("size", &SparseOp::size)
         .def_readonly("shape", &SparseOp::shape)
+        .def_readonly("storage", &SparseOp::storage)
         .def_property_readonly("dtype", [](const SparseOp &) { return py::dtype::of<double>(); });
-    so.def(py::init([](SQuantOp &h, DOCIWfn &w, i64 r, i64 c, bool s) { return new SparseOp(h, w, r, c, s); }),
-           py::arg("ham"), py::arg("wfn"), py::arg("nrow") = -1, py::arg("ncol") = -1, py::arg("symmetric") = true)
-        .def(py::init([](SQuantOp &h, FullCIWfn &w, i64 r, i64 c, bool s) { return new SparseOp(h, w, r, c, s); }),
-             py::arg("ham"), py::arg("wfn"), py::arg("nrow") = -1, py::arg("ncol") = -1, py::arg("symmetric") = true)
-        .def(py::init([](SQuantOp &h, GenCIWfn &w, i64 r, i64 c, bool s) { return new SparseOp(h, w, r, c, s); }),
-             py::arg("ham"), py::arg("wfn"), py::arg("nrow") = -1, py::arg("ncol") = -1, py::arg("symmetric") = true)
+    so.def(py::init([](SQuantOp &h, DOCIWfn &w, i64 r, i64 c, bool s, const std::string &st) {
+                 return new SparseOp(h, w, r, c, s, st);
+             }),
+           py::arg("ham"), py::arg("wfn"), py::arg("nrow") = -1, py::arg("ncol") = -1, py::arg("symmetric") = true,
+             py::arg("storage") = "auto")
+        .def(py::init([](SQuantOp &h, FullCIWfn &w, i64 r, i64 c, bool s, const std::string &st) {
+                 return new SparseOp(h, w, r, c, s, st);
+             }),
+             py::arg("ham"), py::arg("wfn"), py::arg("nrow") = -1, py::arg("ncol") = -1, py::arg("symmetric") = true,
+             py::arg("storage") = "auto")
+        .def(py::init([](SQuantOp &h, GenCIWfn &w, i64 r, i64 c, bool s, const std::string &st) {
+                 return new SparseOp(h, w, r, c, s, st);
+             }),
+             py::arg("ham"), py::arg("wfn"), py::arg("nrow") = -1, py::arg("ncol") = -1, py::arg("symmetric") = true,
+             py::arg("storage") = "auto")
         .def("update", [](SparseOp &o, SQuantOp &h, DOCIWfn &w) { o.update(h, w); }, py::arg("ham"), py::arg("wfn"))
         .def("update", [](SparseOp &o, SQuantOp &h, FullCIWfn &w) { o.update(h, w); }, py::arg("ham"), py::arg("wfn"))
         .def("update", [](SparseOp &o, SQuantOp &h, GenCIWfn &w) { o.update(h, w); }, py::arg("ham"), py::arg("wfn"))
@@ -358,14 +420,26 @@ PYBIND11_MODULE(_pyci, m) {
           },
           py::arg("det"));
     m.def("add_hci", &add_hci<DOCIWfn>, py::arg("ham"), py::arg("wfn"), py::arg("coeffs"), py::arg("eps") = 1.0e-5,
-          py::arg("nthread") = -1);
+          py::arg("nthread") = -1, py::arg("selector") = py::none(), py::arg("pool") = 5.0, py::arg("nadd") = -1,
+          py::arg("score_mode") = 0);
     m.def("add_hci", &add_hci<FullCIWfn>, py::arg("ham"), py::arg("wfn"), py::arg("coeffs"), py::arg("eps") = 1.0e-5,
-          py::arg("nthread") = -1);
+          py::arg("nthread") = -1, py::arg("selector") = py::none(), py::arg("pool") = 5.0, py::arg("nadd") = -1,
+          py::arg("score_mode") = 0);
     m.def("add_hci", &add_hci<GenCIWfn>, py::arg("ham"), py::arg("wfn"), py::arg("coeffs"), py::arg("eps") = 1.0e-5,
-          py::arg("nthread") = -1);
+          py::arg("nthread") = -1, py::arg("selector") = py::none(), py::arg("pool") = 5.0, py::arg("nadd") = -1,
+          py::arg("score_mode") = 0);
     m.def("compute_rdms", [](DOCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));
     m.def("compute_rdms", [](FullCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));
     m.def("compute_rdms", [](GenCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));
+    py::class_<MLPSelector>(m, "mlp_selector")
+        .def(py::init<i64, u64>(), py::arg("nfeat"), py::arg("seed") = 0)
+        .def_readonly("nfeat", &MLPSelector::nfeat)
+        .def_property("params", &MLPSelector::get_params, &MLPSelector::set_params)
+        .def("predict", &MLPSelector::predict, py::arg("wfn"))
+        .def("loss_grad", &MLPSelector::loss_grad, py::arg("wfn"), py::arg("target"))
+        .def("train", &MLPSelector::train, py::arg("wfn"), py::arg("coeffs"), py::arg("steps") = 300,
+             py::arg("batch") = 8192, py::arg("lr") = 1.0e-3, py::arg("cmin") = 1.0e-10, py::arg("seed") = 0);
+    m.attr("MLP_NPARAM") = (i64)PYCI_MLP_NPARAM;
     m.def("device_count", [] {
         int c = 0;
         check(pyci_device_count(&c));

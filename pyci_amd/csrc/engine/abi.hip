@@ -1,5 +1,6 @@
 // extern "C" entry points of include/pyci_engine.h.
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "internal.hpp"
@@ -264,7 +265,14 @@ int pyci_fill_all_dets(pyci_ctx *ctx, int kind, int64_t nbasis, int64_t nup, int
 }
 
 int pyci_op_create(pyci_ham *ham, pyci_wfn *wfn, int64_t nrow, int64_t ncol, int symmetric, pyci_op **out) {
+    return pyci_op_create_ex(ham, wfn, nrow, ncol, symmetric, PYCI_STORAGE_CSR, out);
+}
+
+int pyci_op_create_ex(pyci_ham *ham, pyci_wfn *wfn, int64_t nrow, int64_t ncol, int symmetric, int storage,
+                      pyci_op **out) {
     return guarded([&] {
+        PE_REQUIRE(storage == PYCI_STORAGE_CSR || storage == PYCI_STORAGE_DIRECT || storage == PYCI_STORAGE_AUTO,
+                   PYCI_EINVAL, "unknown storage");
         PE_REQUIRE(ham->ctx == wfn->ctx, PYCI_EINVAL, "ham and wfn belong to different contexts");
         const i64 nbexp = (wfn->kind == PYCI_WFN_GENCI) ? wfn->nbasis : wfn->nbasis;
         PE_REQUIRE(ham->nbasis == nbexp, PYCI_EINVAL, "operator and wave function have different nbasis");
@@ -280,7 +288,15 @@ int pyci_op_create(pyci_ham *ham, pyci_wfn *wfn, int64_t nrow, int64_t ncol, int
             PE_REQUIRE(op->ncol < ((i64)1 << 31), PYCI_EUNSUPPORTED, "ncol must be < 2^31");
             op->symmetric = symmetric ? 1 : 0;
             op->ecore = ham->ecore;
-            op_build_all(op, ham, wfn);
+            const bool direct = storage == PYCI_STORAGE_DIRECT ||
+                                (storage == PYCI_STORAGE_AUTO &&
+                                 direct_applicable(ham, wfn, op->nrow, op->ncol, op->symmetric));
+            if (direct) {
+                op->storage = PYCI_STORAGE_DIRECT;
+                direct_setup(op, ham, wfn);
+            } else {
+                op_build_all(op, ham, wfn);
+            }
         } catch (...) {
             delete op;
             throw;
@@ -289,10 +305,24 @@ int pyci_op_create(pyci_ham *ham, pyci_wfn *wfn, int64_t nrow, int64_t ncol, int
     });
 }
 
+int pyci_op_storage(const pyci_op *op, int *storage) {
+    return guarded([&] { *storage = op->storage; });
+}
+
 int pyci_op_update(pyci_op *op, pyci_ham *ham, pyci_wfn *wfn) {
     return guarded([&] {
         set_device(op->ctx);
         PE_REQUIRE(wfn->ndet < ((i64)1 << 31), PYCI_EUNSUPPORTED, "ncol must be < 2^31");
+        if (op->storage == PYCI_STORAGE_DIRECT) {
+            // unchanged full space: nothing to do; otherwise continue as a CSR operator
+            if (wfn->ndet == op->nrow && direct_applicable(ham, wfn, wfn->ndet, wfn->ndet, op->symmetric)) return;
+            op->storage = PYCI_STORAGE_CSR;
+            op->direct.reset();
+            op->nrow = wfn->ndet;
+            op->ncol = wfn->ndet;
+            op_build_all(op, ham, wfn);
+            return;
+        }
         const i64 old = op->nrow;
         if (op->symmetric) {
             // The lower-triangle export of rows [0, old) is unchanged by new columns
@@ -366,6 +396,7 @@ int pyci_op_info(const pyci_op *op, int64_t *nrow, int64_t *ncol, int64_t *size,
 int pyci_op_export(pyci_op *op, double *data, int64_t *indices, int64_t *indptr) {
     return guarded([&] {
         set_device(op->ctx);
+        direct_ensure_csr(op);
         export_csr(op, data, indices, indptr);
     });
 }
@@ -378,6 +409,7 @@ int pyci_op_get_element(pyci_op *op, int64_t i, int64_t j, double *value) {
             *value = 0.0;
             return;
         }
+        direct_ensure_csr(op);
         *value = get_element(op, i, j);
     });
 }
@@ -479,6 +511,77 @@ int pyci_compute_rdms(pyci_wfn *wfn, const double *coeffs, double *rdm1, double 
     return guarded([&] {
         set_device(wfn->ctx);
         rdms_device(wfn, coeffs, rdm1, rdm2);
+    });
+}
+
+int pyci_mlp_create(pyci_ctx *ctx, int64_t nfeat, uint64_t seed, pyci_mlp **out) {
+    return guarded([&] {
+        PE_REQUIRE(ctx != nullptr && out != nullptr, PYCI_EINVAL, "null argument");
+        set_device(ctx);
+        auto m = std::make_unique<pyci_mlp>();
+        m->ctx = ctx;
+        mlp_init(m.get(), (int)nfeat, seed);
+        *out = m.release();
+    });
+}
+
+int pyci_mlp_destroy(pyci_mlp *m) {
+    return guarded([&] {
+        if (!m) return;
+        set_device(m->ctx);
+        PE_HIP(hipStreamSynchronize(m->ctx->stream));
+        delete m;
+    });
+}
+
+int pyci_mlp_get_params(pyci_mlp *m, float *params) {
+    return guarded([&] {
+        set_device(m->ctx);
+        mlp_get_params(m, params);
+    });
+}
+
+int pyci_mlp_set_params(pyci_mlp *m, const float *params) {
+    return guarded([&] {
+        set_device(m->ctx);
+        mlp_set_params(m, params);
+    });
+}
+
+int pyci_mlp_predict(pyci_mlp *m, int kind, int64_t nbasis, int64_t n, const uint64_t *dets, float *y) {
+    return guarded([&] {
+        PE_REQUIRE(n >= 0, PYCI_EINVAL, "negative count");
+        set_device(m->ctx);
+        mlp_predict(m, kind, nbasis, n, dets, y);
+    });
+}
+
+int pyci_mlp_loss_grad(pyci_mlp *m, int kind, int64_t nbasis, int64_t n, const uint64_t *dets, const float *target,
+                       float *loss, float *grad) {
+    return guarded([&] {
+        set_device(m->ctx);
+        *loss = mlp_loss_grad(m, kind, nbasis, n, dets, target, grad);
+    });
+}
+
+int pyci_mlp_train(pyci_mlp *m, const pyci_wfn *wfn, const double *coeffs, double cmin, int64_t steps,
+                   int64_t batch, double lr, uint64_t seed, float *last_loss) {
+    return guarded([&] {
+        PE_REQUIRE(m->ctx == wfn->ctx, PYCI_EINVAL, "network and wfn belong to different contexts");
+        set_device(m->ctx);
+        const float l = mlp_train(m, wfn, coeffs, cmin, steps, batch, lr, seed);
+        if (last_loss) *last_loss = l;
+    });
+}
+
+int pyci_add_hci_mlp(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs, double eps, pyci_mlp *m, double pool,
+                     int64_t nadd, int score_mode, int64_t *nadded) {
+    return guarded([&] {
+        PE_REQUIRE(ham->ctx == wfn->ctx && m->ctx == wfn->ctx, PYCI_EINVAL,
+                   "ham, wfn and network belong to different contexts");
+        set_device(wfn->ctx);
+        const i64 k = add_hci_mlp_device(ham, wfn, coeffs, eps, m, pool, nadd, score_mode);
+        if (nadded) *nadded = k;
     });
 }
 

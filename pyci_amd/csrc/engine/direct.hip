@@ -1,0 +1,463 @@
+// Matrix-free FullCI operator over the full determinant space in colex order.
+//
+// The reference always stores the CSR (SparseOp::update, sparseop.cpp:186-201)
+// and multiplies with it (perform_op_symm, sparseop.cpp:148-159). For the full
+// FullCI space this operator applies the same Hamiltonian without storing it,
+// in the spin-free form
+//   H = sum_pq k_pq E_pq + 1/2 sum_pqrs (pq|rs) E_pq E_rs,
+//   k_pq = h_pq - 1/2 sum_r (pr|rq),  E_pq = E^alpha_pq + E^beta_pq,
+// with (pq|rs) = two_mo[p][r][q][s]. Determinant J = (Ja, Jb) has index
+// Ja * NB + Jb (colex ranks of its strings), so a vector is a row-major
+// NA x NB matrix. One sigma Y = H X is three passes:
+//   D[kl][J] = sum_K <J|E_kl|K> X[K]                 (gathers over single-excitation tables)
+//   E[ij][J] = 1/2 sum_kl (ij|kl) D[kl][J] + k_ij X[J] (dense product on MFMA, in place)
+//   Y[I]     = sum_ij sum_J <I|E_ij|J> E[ij][J]        (gathers over the same tables)
+// Both D and E are symmetric in their orbital pair ((ij|kl) = (ji|kl) for real
+// orbitals), so pairs are packed as p = i (i + 1) / 2 + j, i >= j, and D holds
+// D_kl + D_lk. Each string's table lists, for k occupied and l virtual or l = k,
+// the rank of the string with k replaced by l, the pair of (k, l) and the sign
+// (-1)^(occupied orbitals strictly between k and l) (phase_single_det,
+// common.cpp:163-194); entries are sorted by pair, so that the D pass writes
+// each D element exactly once by merging the alpha and beta lists.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <type_traits>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace pe {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+// D pass: one thread per determinant J; NV >= nvec vectors (row-major X[J][v]).
+template <int NV, typename T>
+__global__ void __launch_bounds__(256) k_mf_d(i64 nd, i64 NB, int npair, int nea, int neb, const int2 *__restrict__ ta,
+                                              const int2 *__restrict__ tb, int nvec, const T *__restrict__ X,
+                                              T *__restrict__ W, i64 ldw) {
+    const i64 J = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (J >= nd) return;
+    const i64 Ja = J / NB, Jb = J - Ja * NB;
+    const int2 *pa = ta + Ja * nea, *pb = tb + Jb * neb;
+    int ia = 0, ib = 0;
+    int2 ea = pa[0], eb = pb[0];
+    for (int q = 0; q < npair; ++q) {
+        T acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = (T)0;
+        if (ia < nea && (ea.y & 0xffff) == q) {
+            const T s = (ea.y >> 16) ? (T)-1 : (T)1;
+            const T *x = X + ((i64)ea.x * NB + Jb) * nvec;
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                if (v < nvec) acc[v] += s * x[v];
+            if (++ia < nea) ea = pa[ia];
+        }
+        if (ib < neb && (eb.y & 0xffff) == q) {
+            const T s = (eb.y >> 16) ? (T)-1 : (T)1;
+            const T *x = X + (Ja * NB + eb.x) * nvec;
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                if (v < nvec) acc[v] += s * x[v];
+            if (++ib < neb) eb = pb[ib];
+        }
+        T *w = W + (i64)q * ldw + J * nvec;
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            if (v < nvec) w[v] = acc[v];
+    }
+}
+
+// Product pass, in place on W (kpad rows, ldw columns, ldw % 128 == 0):
+//   W[p][c] = 1/2 sum_q V[p][q] W[q][c] + k[p] X[c]      (X[c] = 0 for c >= ncol)
+// V is kpad x kpad (zero padded), MT = kpad / 16 row tiles. 256 threads: each
+// wave owns 32 columns (two 16-column tiles) and all MT row tiles; every column
+// is read completely before the wave writes it. MFMA 16x16x4: lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator r of lane l is
+// D[4 (l / 16) + r][l % 16].
+template <int MT, typename T>
+__global__ void __launch_bounds__(256) k_mf_gemm(i64 ncol, const T *__restrict__ V, const T *__restrict__ kv,
+                                                 const T *__restrict__ X, T *W, i64 ldw) {
+    constexpr int KP = MT * 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const i64 c0 = (i64)blockIdx.x * 128 + wave * 32;
+    typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type acc_t;
+    acc_t acc[MT][2];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+        acc[m][0] = acc_t{};
+        acc[m][1] = acc_t{};
+    }
+#pragma unroll 2
+    for (int k0 = 0; k0 < KP; k0 += 4) {
+        const T b0 = W[(i64)(k0 + lk) * ldw + c0 + lr];
+        const T b1 = W[(i64)(k0 + lk) * ldw + c0 + 16 + lr];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const T a = V[(16 * m + lr) * KP + k0 + lk];
+            if constexpr (sizeof(T) == 8) {
+                acc[m][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[m][0], 0, 0, 0);
+                acc[m][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[m][1], 0, 0, 0);
+            } else {
+                acc[m][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[m][0], 0, 0, 0);
+                acc[m][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[m][1], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const i64 c = c0 + 16 * t + lr;
+        const T x = (c < ncol) ? X[c] : (T)0;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int p = 16 * m + 4 * lk + r;
+                W[(i64)p * ldw + c] = (T)0.5 * acc[m][t][r] + kv[p] * x;
+            }
+    }
+}
+
+// Scatter pass: Y[I][v] = sum over alpha and beta table entries of I.
+template <int NV, typename T>
+__global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int neb, const int2 *__restrict__ ta,
+                                                  const int2 *__restrict__ tb, int nvec, const T *__restrict__ W,
+                                                  i64 ldw, T *__restrict__ Y) {
+    const i64 I = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nd) return;
+    const i64 Ia = I / NB, Ib = I - Ia * NB;
+    T acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = (T)0;
+    const int2 *pa = ta + Ia * nea;
+    for (int t = 0; t < nea; ++t) {
+        const int2 e = pa[t];
+        const T s = (e.y >> 16) ? (T)-1 : (T)1;
+        const T *w = W + (i64)(e.y & 0xffff) * ldw + ((i64)e.x * NB + Ib) * nvec;
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            if (v < nvec) acc[v] += s * w[v];
+    }
+    const int2 *pb = tb + Ib * neb;
+    for (int t = 0; t < neb; ++t) {
+        const int2 e = pb[t];
+        const T s = (e.y >> 16) ? (T)-1 : (T)1;
+        const T *w = W + (i64)(e.y & 0xffff) * ldw + (Ia * NB + e.x) * nvec;
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            if (v < nvec) acc[v] += s * w[v];
+    }
+    T *y = Y + I * nvec;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+        if (v < nvec) y[v] = acc[v];
+}
+
+#define GIX(p, q, r, s) ((((p) * n + (q)) * n + (r)) * n + (s))
+
+// Diagonal <J|H|J> with the summation order of the CSR build's diagonal.
+__global__ void k_mf_diag(i64 nd, i64 NB, const u64 *__restrict__ sa, const u64 *__restrict__ sb, HamView H,
+                          double *__restrict__ diag) {
+    const i64 J = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (J >= nd) return;
+    const int n = H.n;
+    const double *one = H.one, *two = H.two;
+    const i64 Ja = J / NB;
+    const u64 up = sa[Ja], dn = sb[J - Ja * NB];
+    double v2 = 0.0;
+    for (u64 x = up; x; x &= x - 1) {
+        const int ii = __builtin_ctzll(x);
+        v2 += one[(n + 1) * ii];
+        for (u64 y = x & (x - 1); y; y &= y - 1) {
+            const int kk = __builtin_ctzll(y);
+            v2 += two[GIX(ii, kk, ii, kk)] - two[GIX(ii, kk, kk, ii)];
+        }
+        for (u64 y = dn; y; y &= y - 1) {
+            const int kk = __builtin_ctzll(y);
+            v2 += two[GIX(ii, kk, ii, kk)];
+        }
+    }
+    for (u64 x = dn; x; x &= x - 1) {
+        const int ii = __builtin_ctzll(x);
+        v2 += one[(n + 1) * ii];
+        for (u64 y = x & (x - 1); y; y &= y - 1) {
+            const int kk = __builtin_ctzll(y);
+            v2 += two[GIX(ii, kk, ii, kk)] - two[GIX(ii, kk, kk, ii)];
+        }
+    }
+    diag[J] = v2;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static std::vector<u64> colex_strings(int n, int k) {
+    std::vector<u64> s;
+    if (k == 0) {
+        s.push_back(0);
+        return s;
+    }
+    const u64 last = ((1ULL << k) - 1ULL) << (n - k);
+    u64 v = (1ULL << k) - 1ULL;
+    for (;;) {
+        s.push_back(v);
+        if (v == last) break;
+        const u64 t = v | (v - 1ULL);  // next integer with the same popcount (colex successor)
+        v = (t + 1ULL) | (((~t & (t + 1ULL)) - 1ULL) >> (__builtin_ctzll(v) + 1));
+    }
+    return s;
+}
+
+static i64 rank_host(u64 det, const std::vector<i64> &binom) {
+    i64 r = 0;
+    int k = 0;
+    while (det) {
+        const int p = __builtin_ctzll(det);
+        ++k;
+        r += binom[(size_t)p * 65 + k];
+        det &= det - 1;
+    }
+    return r;
+}
+
+static std::vector<int2> string_table(int n, const std::vector<u64> &S, int ne, const std::vector<i64> &binom) {
+    std::vector<int2> t(S.size() * (size_t)ne);
+    std::vector<std::pair<int, int2>> row;
+    for (size_t r = 0; r < S.size(); ++r) {
+        const u64 s = S[r];
+        row.clear();
+        for (int k = 0; k < n; ++k) {
+            if (!((s >> k) & 1ULL)) continue;
+            for (int l = 0; l < n; ++l) {
+                if (l != k && ((s >> l) & 1ULL)) continue;
+                u64 K = s;
+                int neg = 0;
+                if (l != k) {
+                    K = s ^ (1ULL << k) ^ (1ULL << l);
+                    const int lo = std::min(k, l), hi = std::max(k, l);
+                    const u64 between = ((hi >= 64) ? ~0ULL : ((1ULL << hi) - 1ULL)) & ~((1ULL << (lo + 1)) - 1ULL);
+                    neg = __builtin_popcountll(s & between) & 1;
+                }
+                const int i = std::max(k, l), j = std::min(k, l);
+                const int p = i * (i + 1) / 2 + j;
+                row.push_back({p, int2{(int)rank_host(K, binom), p | (neg << 16)}});
+            }
+        }
+        std::sort(row.begin(), row.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        for (int e = 0; e < ne; ++e) t[r * ne + e] = row[(size_t)e].second;
+    }
+    return t;
+}
+
+bool direct_applicable(const pyci_ham *ham, const pyci_wfn *wfn, i64 nrow, i64 ncol, int symmetric) {
+    if (wfn->kind != PYCI_WFN_FULLCI || !wfn->colex || !symmetric) return false;
+    if (nrow != wfn->ndet || ncol != wfn->ndet || wfn->ndet < 2) return false;
+    const i64 n = ham->nbasis;
+    if (n > 15) return false;  // pair dimension <= 128: MT <= 8 row tiles
+    if (wfn->nup == 0 || wfn->ndn == 0) return false;
+    return true;
+}
+
+void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn) {
+    PE_REQUIRE(direct_applicable(ham, wfn, op->nrow, op->ncol, op->symmetric), PYCI_EUNSUPPORTED,
+               "the matrix-free operator needs the full FullCI space in colex order, a square symmetric operator, "
+               "nbasis <= 15 and both spins occupied");
+    hipStream_t s = op->ctx->stream;
+    auto D = std::make_unique<DirectOp>();
+    const int n = (int)ham->nbasis;
+    D->n = n;
+    D->nup = (int)wfn->nup;
+    D->ndn = (int)wfn->ndn;
+    D->ecore = ham->ecore;
+    D->npair = n * (n + 1) / 2;
+    D->kpad = (D->npair + 15) / 16 * 16;
+    std::vector<i64> binom(65 * 65);
+    for (int a = 0; a <= 64; ++a)
+        for (int b = 0; b <= 64; ++b) binom[(size_t)a * 65 + b] = binom_host(a, b);
+    const std::vector<u64> sa = colex_strings(n, D->nup), sb = colex_strings(n, D->ndn);
+    D->na = (i64)sa.size();
+    D->nbs = (i64)sb.size();
+    PE_REQUIRE(D->na * D->nbs == wfn->ndet, PYCI_EINVAL, "wave function is not the full FullCI space");
+    D->nea = D->nup * (n - D->nup + 1);
+    D->neb = D->ndn * (n - D->ndn + 1);
+    const std::vector<int2> ta = string_table(n, sa, D->nea, binom), tb = string_table(n, sb, D->neb, binom);
+    D->tab_a.alloc(ta.size());
+    D->tab_b.alloc(tb.size());
+    D->str_a.alloc(sa.size());
+    D->str_b.alloc(sb.size());
+    PE_HIP(hipMemcpyAsync(D->tab_a.p, ta.data(), sizeof(int2) * ta.size(), hipMemcpyHostToDevice, s));
+    PE_HIP(hipMemcpyAsync(D->tab_b.p, tb.data(), sizeof(int2) * tb.size(), hipMemcpyHostToDevice, s));
+    PE_HIP(hipMemcpyAsync(D->str_a.p, sa.data(), sizeof(u64) * sa.size(), hipMemcpyHostToDevice, s));
+    PE_HIP(hipMemcpyAsync(D->str_b.p, sb.data(), sizeof(u64) * sb.size(), hipMemcpyHostToDevice, s));
+    // integrals: host copies for the pair matrices, device copies for the diagonal and a later CSR
+    const size_t n2 = (size_t)n * n, n4 = n2 * n2;
+    std::vector<double> one(n2), two(n4);
+    PE_HIP(hipMemcpyAsync(one.data(), ham->one_mo.p, sizeof(double) * n2, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipMemcpyAsync(two.data(), ham->two_mo.p, sizeof(double) * n4, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    D->one_mo.alloc(n2);
+    D->two_mo.alloc(n4);
+    PE_HIP(hipMemcpyAsync(D->one_mo.p, one.data(), sizeof(double) * n2, hipMemcpyHostToDevice, s));
+    PE_HIP(hipMemcpyAsync(D->two_mo.p, two.data(), sizeof(double) * n4, hipMemcpyHostToDevice, s));
+    const int KP = D->kpad;
+    std::vector<double> V((size_t)KP * KP, 0.0), kv((size_t)KP, 0.0);
+    auto g = [&](int p, int q, int r, int t) { return two[(((size_t)p * n + q) * n + r) * n + t]; };
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            const int p = i * (i + 1) / 2 + j;
+            double kk = one[(size_t)i * n + j];
+            for (int r = 0; r < n; ++r) kk -= 0.5 * g(i, r, r, j);  // (ir|rj) = <ir|rj>
+            kv[(size_t)p] = kk;
+            for (int k = 0; k < n; ++k)
+                for (int l = 0; l <= k; ++l) V[(size_t)p * KP + k * (k + 1) / 2 + l] = g(i, k, j, l);  // (ij|kl)
+        }
+    std::vector<float> V32(V.begin(), V.end()), kv32(kv.begin(), kv.end());
+    D->V64.alloc(V.size());
+    D->k64.alloc(kv.size());
+    D->V32.alloc(V.size());
+    D->k32.alloc(kv.size());
+    PE_HIP(hipMemcpyAsync(D->V64.p, V.data(), sizeof(double) * V.size(), hipMemcpyHostToDevice, s));
+    PE_HIP(hipMemcpyAsync(D->k64.p, kv.data(), sizeof(double) * kv.size(), hipMemcpyHostToDevice, s));
+    PE_HIP(hipMemcpyAsync(D->V32.p, V32.data(), sizeof(float) * V32.size(), hipMemcpyHostToDevice, s));
+    PE_HIP(hipMemcpyAsync(D->k32.p, kv32.data(), sizeof(float) * kv32.size(), hipMemcpyHostToDevice, s));
+    // diagonal (Davidson preconditioner, nrow == 1 solve, diagonal())
+    const i64 nd = wfn->ndet;
+    op->diag.alloc((size_t)nd);
+    HamView H{n, D->one_mo.p, D->two_mo.p};
+    k_mf_diag<<<grid_for(nd, 256), 256, 0, s>>>(nd, D->nbs, D->str_a.p, D->str_b.p, H, op->diag.p);
+    PE_LAUNCH_CHECK();
+    PE_HIP(hipStreamSynchronize(s));
+    // reference-format size of the full-space CSR: every single and double
+    // excitation is present, so each row has the same count
+    const i64 na_ = D->nup, va = n - D->nup, nb_ = D->ndn, vb = n - D->ndn;
+    const i64 per_row = na_ * va + nb_ * vb + (na_ * (na_ - 1) / 2) * (va * (va - 1) / 2) +
+                        (nb_ * (nb_ - 1) / 2) * (vb * (vb - 1) / 2) + na_ * va * nb_ * vb + 1;
+    op->size_ref = (per_row - 1) * nd / 2 + nd;
+    op->nnz = 0;
+    op->direct = std::move(D);
+}
+
+template <typename T>
+static T *direct_workspace(DirectOp *D, i64 ldw, hipStream_t s);
+
+template <>
+double *direct_workspace<double>(DirectOp *D, i64 ldw, hipStream_t s) {
+    if (D->ldw64 < ldw) {
+        D->work64.alloc((size_t)D->kpad * ldw);
+        PE_HIP(hipMemsetAsync(D->work64.p, 0, sizeof(double) * D->kpad * ldw, s));  // pad rows / columns stay zero
+        D->ldw64 = ldw;
+    }
+    return D->work64.p;
+}
+
+template <>
+float *direct_workspace<float>(DirectOp *D, i64 ldw, hipStream_t s) {
+    if (D->ldw32 < ldw) {
+        D->work32.alloc((size_t)D->kpad * ldw);
+        PE_HIP(hipMemsetAsync(D->work32.p, 0, sizeof(float) * D->kpad * ldw, s));
+        D->ldw32 = ldw;
+    }
+    return D->work32.p;
+}
+
+template <int MT, typename T>
+static void launch_gemm(i64 ncol, const T *V, const T *kv, const T *X, T *W, i64 ldw, hipStream_t s) {
+    // only the column tiles that hold the ncol live columns (ldw >= their end)
+    k_mf_gemm<MT, T><<<(unsigned)((ncol + 127) / 128), 256, 0, s>>>(ncol, V, kv, X, W, ldw);
+    PE_LAUNCH_CHECK();
+}
+
+template <int NV, typename T>
+static void launch_passes(const DirectOp *D, i64 nd, int nvec, const T *X, T *W, i64 ldw, T *Y, const T *V,
+                          const T *kv, hipStream_t s) {
+    const unsigned g = grid_for(nd, 256);
+    k_mf_d<NV, T><<<g, 256, 0, s>>>(nd, D->nbs, D->npair, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, X, W, ldw);
+    PE_LAUNCH_CHECK();
+    const i64 ncol = nd * nvec;
+    switch (D->kpad / 16) {
+        case 1: launch_gemm<1, T>(ncol, V, kv, X, W, ldw, s); break;
+        case 2: launch_gemm<2, T>(ncol, V, kv, X, W, ldw, s); break;
+        case 3: launch_gemm<3, T>(ncol, V, kv, X, W, ldw, s); break;
+        case 4: launch_gemm<4, T>(ncol, V, kv, X, W, ldw, s); break;
+        case 5: launch_gemm<5, T>(ncol, V, kv, X, W, ldw, s); break;
+        case 6: launch_gemm<6, T>(ncol, V, kv, X, W, ldw, s); break;
+        case 7: launch_gemm<7, T>(ncol, V, kv, X, W, ldw, s); break;
+        case 8: launch_gemm<8, T>(ncol, V, kv, X, W, ldw, s); break;
+        default: throw EngineError(PYCI_EUNSUPPORTED, "pair dimension too large for the matrix-free operator");
+    }
+    k_mf_sigma<NV, T><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, W, ldw, Y);
+    PE_LAUNCH_CHECK();
+}
+
+template <typename T>
+void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s) {
+    DirectOp *D = op->direct.get();
+    PE_REQUIRE(nvec >= 1 && nvec <= 16, PYCI_EUNSUPPORTED, "the matrix-free operator takes 1..16 vectors");
+    const i64 nd = op->nrow;
+    const i64 ldw = (nd * nvec + 127) / 128 * 128;
+    T *W = direct_workspace<T>(D, ldw, s);
+    const T *V = std::is_same<T, double>::value ? (const T *)D->V64.p : (const T *)D->V32.p;
+    const T *kv = std::is_same<T, double>::value ? (const T *)D->k64.p : (const T *)D->k32.p;
+    const i64 ldw_used = std::is_same<T, double>::value ? D->ldw64 : D->ldw32;
+    if (nvec == 1) launch_passes<1, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s);
+    else if (nvec == 2) launch_passes<2, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s);
+    else if (nvec <= 4) launch_passes<4, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s);
+    else if (nvec <= 8) launch_passes<8, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s);
+    else launch_passes<16, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s);
+}
+
+template void direct_sigma<double>(const pyci_op *, int, const double *, double *, hipStream_t);
+template void direct_sigma<float>(const pyci_op *, int, const float *, float *, hipStream_t);
+
+// Materialises the CSR of a matrix-free operator (for exports and element
+// queries), from a regenerated full-space wave function and the stored integrals.
+void direct_ensure_csr(pyci_op *op) {
+    DirectOp *D = op->direct.get();
+    if (!D || D->csr_ready) return;
+    pyci_ctx *ctx = op->ctx;
+    pyci_ham ham;
+    ham.ctx = ctx;
+    ham.nbasis = D->n;
+    ham.ecore = D->ecore;
+    ham.one_mo.alloc((size_t)D->n * D->n);
+    ham.two_mo.alloc((size_t)D->n * D->n * D->n * D->n);
+    hipStream_t s = ctx->stream;
+    PE_HIP(hipMemcpyAsync(ham.one_mo.p, D->one_mo.p, sizeof(double) * D->n * D->n, hipMemcpyDeviceToDevice, s));
+    PE_HIP(hipMemcpyAsync(ham.two_mo.p, D->two_mo.p, sizeof(double) * D->n * D->n * D->n * D->n,
+                          hipMemcpyDeviceToDevice, s));
+    pyci_wfn wfn;
+    wfn.ctx = ctx;
+    wfn.kind = PYCI_WFN_FULLCI;
+    wfn.nbasis = D->n;
+    wfn.nup = D->nup;
+    wfn.ndn = D->ndn;
+    wfn.nkey = 2;
+    wfn.nsd = D->nbs;
+    wfn.ndet = op->nrow;
+    wfn.dets.alloc((size_t)(2 * op->nrow));
+    fill_all_dets_device(ctx, PYCI_WFN_FULLCI, D->n, D->nup, D->ndn, wfn.dets.p, 1);
+    wfn_build_index(&wfn);
+    DevBuf<i64> rp, lc;
+    DevBuf<int32_t> cols;
+    DevBuf<double> vals, dg;
+    i64 nnz = 0, low = 0;
+    build_rows(op, &ham, &wfn, 0, op->nrow, op->ncol, rp, cols, vals, lc, dg, nnz, low);
+    PE_HIP(hipStreamSynchronize(s));
+    op->rowptr = std::move(rp);
+    op->lowcnt = std::move(lc);
+    op->cols = std::move(cols);
+    op->vals = std::move(vals);
+    op->vals32.release();
+    op->nnz = nnz;
+    op->size_ref = low;
+    op->segments.assign(1, {op->nrow, op->ncol});
+    D->csr_ready = true;
+}
+
+}  // namespace pe

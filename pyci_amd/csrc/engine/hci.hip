@@ -17,6 +17,10 @@
 // One wavefront handles one source row; the outer (i, j) loops run uniformly
 // and lanes cover the inner excitation block, so ballot compaction preserves
 // the reference order.
+//
+// The same passes also build a candidate pool for the learned selector: the
+// insertion threshold is then a looser eps_pool, and each candidate records its
+// heat-bath score max_i |H_ji c_i| and whether plain add_hci(eps) takes it.
 #include <algorithm>
 
 #include "internal.hpp"
@@ -33,6 +37,8 @@ struct NewSet {
     u64 mask;
     unsigned long long *unique;  // number of claimed slots
     int *overflow;
+    u64 *score = nullptr;  // pool only: max |H_ji c_i| as double bits (nonnegative, so integer max)
+    int *plain = nullptr;  // pool only: 1 when some source passes the plain threshold
 };
 
 __device__ inline u64 set_key(const WfnView &w, u64 a, u64 b) {
@@ -40,8 +46,9 @@ __device__ inline u64 set_key(const WfnView &w, u64 a, u64 b) {
     return a;
 }
 
-// Inserts key with position p (keeps the minimum). Claimed slots are counted.
-__device__ inline void set_insert(const NewSet &S, u64 key, u64 p) {
+// Inserts key with position p (keeps the minimum); returns the slot, or ~0 on
+// overflow. Claimed slots are counted.
+__device__ inline u64 set_insert(const NewSet &S, u64 key, u64 p) {
     u64 h = mix64(key) & S.mask;
     for (u64 step = 0; step <= S.mask; ++step) {
         unsigned long long old = atomicCAS((unsigned long long *)&S.keys[h], (unsigned long long)EMPTY_KEY,
@@ -49,11 +56,23 @@ __device__ inline void set_insert(const NewSet &S, u64 key, u64 p) {
         if (old == EMPTY_KEY) atomicAdd(S.unique, 1ULL);
         if (old == EMPTY_KEY || old == key) {
             atomicMin((unsigned long long *)&S.pos[h], (unsigned long long)p);
-            return;
+            return h;
         }
         h = (h + 1) & S.mask;
     }
     atomicOr(S.overflow, 1);
+    return ~0ULL;
+}
+
+__device__ inline u64 set_find_slot(const NewSet &S, u64 key) {
+    u64 h = mix64(key) & S.mask;
+    for (u64 step = 0; step <= S.mask; ++step) {
+        u64 k = S.keys[h];
+        if (k == key) return h;
+        if (k == EMPTY_KEY) return ~0ULL;
+        h = (h + 1) & S.mask;
+    }
+    return ~0ULL;
 }
 
 __device__ inline u64 set_find_pos(const NewSet &S, u64 key) {
@@ -71,9 +90,9 @@ __device__ inline u64 set_find_pos(const NewSet &S, u64 key) {
 
 // PASS 1: count, 2: insert, 3: owners per row, 4: write owners.
 template <int KIND, int PASS>
-__global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double *coeffs, double eps, i64 rows_stride,
-                                             NewSet S, unsigned long long *count, i64 *row_cnt, const i64 *row_off,
-                                             u64 *out) {
+__global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double *coeffs, double eps, double eps_plain,
+                                             i64 rows_stride, NewSet S, unsigned long long *count, i64 *row_cnt,
+                                             const i64 *row_off, u64 *out) {
     const int lane = threadIdx.x & 63;
     const i64 row = (i64)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= w.ndet) return;
@@ -83,7 +102,8 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
     const u64 up = w.dets[(i64)w.nkey * row];
     const u64 dn = (KIND == PYCI_WFN_FULLCI) ? w.dets[2 * row + 1] : 0ULL;
     const int nou = __popcll(up), nvu = n - nou, nod = __popcll(dn), nvd = n - nod;
-    const double eps_i = eps / fabs(coeffs[row]);
+    const double cabs = fabs(coeffs[row]);
+    const double eps_i = eps / cabs, eps_plain_i = eps_plain / cabs;
     // lane q holds the q-th occupied / virtual orbital of each string
     int tmp;
     auto nth = [](u64 x, int t) {
@@ -110,7 +130,14 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
         if (PASS == 1) {
             local += __popcll(__ballot(pass));
         } else if (PASS == 2) {
-            if (pass) set_insert(S, key, pos);
+            if (pass) {
+                const u64 slot = set_insert(S, key, pos);
+                if (S.score != nullptr && slot != ~0ULL) {
+                    atomicMax((unsigned long long *)&S.score[slot],
+                              (unsigned long long)__double_as_longlong(fabs(val) * cabs));
+                    if (fabs(val) > eps_plain_i) atomicOr(&S.plain[slot], 1);
+                }
+            }
         } else {
             const bool own = pass && (set_find_pos(S, key) == pos);
             const u64 b = __ballot(own);
@@ -238,33 +265,50 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
     }
 }
 
-__global__ void k_set_clear(u64 *keys, u64 *pos, i64 n) {
+__global__ void k_set_clear(u64 *keys, u64 *pos, u64 *score, int *plain, i64 n) {
     for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
         keys[i] = EMPTY_KEY;
         pos[i] = ~0ULL;
+        if (score) {
+            score[i] = 0ULL;
+            plain[i] = 0;
+        }
     }
 }
 
 template <int PASS>
-static void launch_hci(int kind, const WfnView &w, const HamView &H, const double *c, double eps, i64 stride,
-                       const NewSet &S, unsigned long long *count, i64 *rc, const i64 *ro, u64 *out, hipStream_t s) {
+static void launch_hci(int kind, const WfnView &w, const HamView &H, const double *c, double eps, double eps_plain,
+                       i64 stride, const NewSet &S, unsigned long long *count, i64 *rc, const i64 *ro, u64 *out,
+                       hipStream_t s) {
     unsigned g = grid_for(w.ndet, 4);
     if (kind == PYCI_WFN_DOCI)
-        k_hci<PYCI_WFN_DOCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, stride, S, count, rc, ro, out);
+        k_hci<PYCI_WFN_DOCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count, rc, ro, out);
     else if (kind == PYCI_WFN_FULLCI)
-        k_hci<PYCI_WFN_FULLCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, stride, S, count, rc, ro, out);
+        k_hci<PYCI_WFN_FULLCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count, rc, ro, out);
     else
-        k_hci<PYCI_WFN_GENCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, stride, S, count, rc, ro, out);
+        k_hci<PYCI_WFN_GENCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count, rc, ro, out);
     PE_LAUNCH_CHECK();
 }
 
 void wfn_index_append(pyci_wfn *w, i64 first);
 
-i64 add_hci_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, double eps) {
+// Passes 1-3 of the selection: the set of new candidates (|H_ji| > eps/|c_i|,
+// not in the wave function) with their first-occurrence positions, and the
+// exclusive-scanned per-row owner counts in rc (nnew = total).
+struct HciRun {
+    DevBuf<u64> keys, pos, score;
+    DevBuf<int> plain, ovf;
+    DevBuf<unsigned long long> cnt;
+    DevBuf<i64> rc;
+    NewSet S{};
+    i64 stride = 0, nnew = 0;
+};
+
+static void hci_collect(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double eps, double eps_plain, bool pool,
+                        HciRun &R) {
     pyci_ctx *ctx = wfn->ctx;
     hipStream_t s = ctx->stream;
     const i64 nd = wfn->ndet;
-    if (nd == 0) return 0;
     PE_REQUIRE(ham->nbasis == wfn->nbasis, PYCI_EINVAL, "operator and wave function have different nbasis");
     if (wfn->kind == PYCI_WFN_FULLCI) {
         long double tot = (long double)binom_host(wfn->nbasis, wfn->nup) * (long double)binom_host(wfn->nbasis, wfn->ndn);
@@ -272,49 +316,68 @@ i64 add_hci_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, doub
     }
     // positions: row * stride + in-row index; stride >= candidates per row
     const i64 no = wfn->nup, nv = wfn->nbasis - wfn->nup, ndo = wfn->ndn, ndv = wfn->nbasis - wfn->ndn;
-    i64 stride = no * nv + no * nv * ndo * ndv + (no * (no - 1) / 2) * (nv * (nv - 1) / 2) + ndo * ndv +
-                 (ndo * (ndo - 1) / 2) * (ndv * (ndv - 1) / 2) + 1;
-    PE_REQUIRE((long double)stride * (long double)nd < 1.8e19L, PYCI_EUNSUPPORTED, "selection too large");
-    DevBuf<double> dc((size_t)nd);
-    PE_HIP(hipMemcpyAsync(dc.p, coeffs_host, sizeof(double) * nd, hipMemcpyHostToDevice, s));
-    WfnView w = view_of(wfn);
-    HamView H = view_of(ham);
-    DevBuf<unsigned long long> cnt(2);
-    DevBuf<int> ovf(1);
-    PE_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * 2, s));
-    NewSet S{nullptr, nullptr, 0, cnt.p + 1, ovf.p};
-    launch_hci<1>(wfn->kind, w, H, dc.p, eps, stride, S, cnt.p, nullptr, nullptr, nullptr, s);
+    R.stride = no * nv + no * nv * ndo * ndv + (no * (no - 1) / 2) * (nv * (nv - 1) / 2) + ndo * ndv +
+               (ndo * (ndo - 1) / 2) * (ndv * (ndv - 1) / 2) + 1;
+    PE_REQUIRE((long double)R.stride * (long double)nd < 1.8e19L, PYCI_EUNSUPPORTED, "selection too large");
+    const WfnView w = view_of(wfn);
+    const HamView H = view_of(ham);
+    R.cnt.alloc(2);
+    R.ovf.alloc(1);
+    PE_HIP(hipMemsetAsync(R.cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    R.S = NewSet{nullptr, nullptr, 0, R.cnt.p + 1, R.ovf.p};
+    launch_hci<1>(wfn->kind, w, H, dc, eps, eps_plain, R.stride, R.S, R.cnt.p, nullptr, nullptr, nullptr, s);
     unsigned long long upper = 0;
-    PE_HIP(hipMemcpyAsync(&upper, cnt.p, sizeof(upper), hipMemcpyDeviceToHost, s));
+    PE_HIP(hipMemcpyAsync(&upper, R.cnt.p, sizeof(upper), hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
-    if (upper == 0) return 0;
+    R.nnew = 0;
+    if (upper == 0) return;
     // hash set capacity: 2x the candidate count, capped; grown on overflow
     i64 cap = 1024;
     const i64 want = (i64)std::min<unsigned long long>(upper, 1ULL << 29);
     while (cap < 2 * want) cap <<= 1;
-    DevBuf<u64> keys, pos;
     for (;;) {
-        keys.alloc((size_t)cap);
-        pos.alloc((size_t)cap);
-        k_set_clear<<<4096, 256, 0, s>>>(keys.p, pos.p, cap);
+        R.keys.alloc((size_t)cap);
+        R.pos.alloc((size_t)cap);
+        if (pool) {
+            R.score.alloc((size_t)cap);
+            R.plain.alloc((size_t)cap);
+        }
+        k_set_clear<<<4096, 256, 0, s>>>(R.keys.p, R.pos.p, pool ? R.score.p : nullptr, pool ? R.plain.p : nullptr,
+                                         cap);
         PE_LAUNCH_CHECK();
-        PE_HIP(hipMemsetAsync(cnt.p + 1, 0, sizeof(unsigned long long), s));
-        PE_HIP(hipMemsetAsync(ovf.p, 0, sizeof(int), s));
-        S = NewSet{keys.p, pos.p, (u64)(cap - 1), cnt.p + 1, ovf.p};
-        launch_hci<2>(wfn->kind, w, H, dc.p, eps, stride, S, nullptr, nullptr, nullptr, nullptr, s);
+        PE_HIP(hipMemsetAsync(R.cnt.p + 1, 0, sizeof(unsigned long long), s));
+        PE_HIP(hipMemsetAsync(R.ovf.p, 0, sizeof(int), s));
+        R.S = NewSet{R.keys.p, R.pos.p, (u64)(cap - 1), R.cnt.p + 1, R.ovf.p};
+        if (pool) {
+            R.S.score = R.score.p;
+            R.S.plain = R.plain.p;
+        }
+        launch_hci<2>(wfn->kind, w, H, dc, eps, eps_plain, R.stride, R.S, nullptr, nullptr, nullptr, nullptr, s);
         unsigned long long uniq = 0;
         int of = 0;
-        PE_HIP(hipMemcpyAsync(&uniq, cnt.p + 1, sizeof(uniq), hipMemcpyDeviceToHost, s));
-        PE_HIP(hipMemcpyAsync(&of, ovf.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        PE_HIP(hipMemcpyAsync(&uniq, R.cnt.p + 1, sizeof(uniq), hipMemcpyDeviceToHost, s));
+        PE_HIP(hipMemcpyAsync(&of, R.ovf.p, sizeof(int), hipMemcpyDeviceToHost, s));
         PE_HIP(hipStreamSynchronize(s));
         if (!of && 2 * (i64)uniq <= cap) break;
         PE_REQUIRE(cap < ((i64)1 << 34), PYCI_ENOMEM, "selection hash set too large");
         cap <<= 2;
     }
-    DevBuf<i64> rc((size_t)(nd + 1));
-    PE_HIP(hipMemsetAsync(rc.p, 0, sizeof(i64) * (nd + 1), s));
-    launch_hci<3>(wfn->kind, w, H, dc.p, eps, stride, S, nullptr, rc.p, nullptr, nullptr, s);
-    const i64 nnew = exclusive_scan(ctx, rc.p, nd + 1);
+    R.rc.alloc((size_t)(nd + 1));
+    PE_HIP(hipMemsetAsync(R.rc.p, 0, sizeof(i64) * (nd + 1), s));
+    launch_hci<3>(wfn->kind, w, H, dc, eps, eps_plain, R.stride, R.S, nullptr, R.rc.p, nullptr, nullptr, s);
+    R.nnew = exclusive_scan(ctx, R.rc.p, nd + 1);
+}
+
+i64 add_hci_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, double eps) {
+    pyci_ctx *ctx = wfn->ctx;
+    hipStream_t s = ctx->stream;
+    const i64 nd = wfn->ndet;
+    if (nd == 0) return 0;
+    DevBuf<double> dc((size_t)nd);
+    PE_HIP(hipMemcpyAsync(dc.p, coeffs_host, sizeof(double) * nd, hipMemcpyHostToDevice, s));
+    HciRun R;
+    hci_collect(ham, wfn, dc.p, eps, eps, false, R);
+    const i64 nnew = R.nnew;
     if (nnew == 0) return 0;
     // grow the device det array, then write the new determinants in place
     const i64 need = (nd + nnew) * wfn->nkey;
@@ -323,14 +386,55 @@ i64 add_hci_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, doub
         PE_HIP(hipMemcpyAsync(nb.p, wfn->dets.p, sizeof(u64) * nd * wfn->nkey, hipMemcpyDeviceToDevice, s));
         PE_HIP(hipStreamSynchronize(s));
         wfn->dets = std::move(nb);
-        w = view_of(wfn);
     }
-    launch_hci<4>(wfn->kind, w, H, dc.p, eps, stride, S, nullptr, nullptr, rc.p, wfn->dets.p + nd * wfn->nkey, s);
+    launch_hci<4>(wfn->kind, view_of(wfn), view_of(ham), dc.p, eps, eps, R.stride, R.S, nullptr, nullptr, R.rc.p,
+                  wfn->dets.p + nd * wfn->nkey, s);
     PE_HIP(hipStreamSynchronize(s));
     wfn->ndet = nd + nnew;
     wfn_index_append(wfn, nd);
     PE_HIP(hipStreamSynchronize(s));
     return nnew;
+}
+
+// dst[i] = src[i] for i < n, dst[n] = 0 (input of an exclusive scan).
+__global__ void k_int_to_i64(const int *src, i64 n, i64 *dst) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) dst[i] = (i < n) ? (i64)src[i] : 0;
+}
+
+// Score and plain flag of each pooled candidate, looked up by its set key.
+__global__ void k_pool_scores(WfnView w, NewSet S, const u64 *cand, i64 n, double *hb, int *plain) {
+    const i64 o = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    const u64 a = cand[(i64)w.nkey * o];
+    const u64 b = (w.nkey == 2) ? cand[2 * o + 1] : 0ULL;
+    const u64 slot = set_find_slot(S, set_key(w, a, b));
+    hb[o] = (slot == ~0ULL) ? 0.0 : __longlong_as_double((long long)S.score[slot]);
+    plain[o] = (slot == ~0ULL) ? 0 : S.plain[slot];
+}
+
+void hci_pool_device(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double eps, double eps_pool, HciPool &P) {
+    hipStream_t s = wfn->ctx->stream;
+    P.n = 0;
+    P.nplain = 0;
+    if (wfn->ndet == 0) return;
+    HciRun R;
+    hci_collect(ham, wfn, dc, eps_pool, eps, true, R);
+    if (R.nnew == 0) return;
+    P.n = R.nnew;
+    P.dets.alloc((size_t)(R.nnew * wfn->nkey));
+    P.hb.alloc((size_t)R.nnew);
+    P.plain.alloc((size_t)R.nnew);
+    const WfnView w = view_of(wfn);
+    launch_hci<4>(wfn->kind, w, view_of(ham), dc, eps_pool, eps, R.stride, R.S, nullptr, nullptr, R.rc.p, P.dets.p, s);
+    k_pool_scores<<<grid_for(P.n, 256), 256, 0, s>>>(w, R.S, P.dets.p, P.n, P.hb.p, P.plain.p);
+    PE_LAUNCH_CHECK();
+    // number of candidates plain add_hci(eps) appends
+    DevBuf<i64> cnt((size_t)(P.n + 1));
+    k_int_to_i64<<<grid_for(P.n + 1, 256), 256, 0, s>>>(P.plain.p, P.n, cnt.p);
+    PE_LAUNCH_CHECK();
+    P.nplain = exclusive_scan(wfn->ctx, cnt.p, P.n + 1);
+    PE_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace pe

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -124,6 +125,39 @@ struct pyci_wfn {
     pe::i64 hcap = 0;
 };
 
+namespace pe {
+// Matrix-free FullCI operator state (direct.hip).
+struct DirectOp {
+    int n = 0, nup = 0, ndn = 0, npair = 0, kpad = 0, nea = 0, neb = 0;
+    i64 na = 0, nbs = 0;         // alpha / beta string counts
+    double ecore = 0.0;
+    bool csr_ready = false;      // the CSR arrays of the operator have been materialised
+    DevBuf<int2> tab_a, tab_b;   // [string][entry] {rank of the excited string, pair | negative << 16}
+    DevBuf<u64> str_a, str_b;    // strings in colex order
+    DevBuf<double> one_mo, two_mo, V64, k64;
+    DevBuf<float> V32, k32;
+    DevBuf<double> work64;       // D / E passes, kpad rows x ldw64 columns
+    DevBuf<float> work32;
+    i64 ldw64 = 0, ldw32 = 0;
+};
+}  // namespace pe
+
+// Learned selector (mlp.hip): FP32 master parameters + Adam state, bf16
+// operand copies of the weights, and workspaces sized for the largest batch.
+struct pyci_mlp {
+    pyci_ctx *ctx = nullptr;
+    int nfeat = 0;
+    long long adam_t = 0;
+    pe::DevBuf<float> params, grads, adam_m, adam_v;
+    pe::DevBuf<uint16_t> w1t;  // [256][64]  layer-1 operand: row = hidden unit, k = input feature
+    pe::DevBuf<uint16_t> w2t;  // [256][256] layer-2 operand: row = output unit f2, k = f1
+    pe::DevBuf<uint16_t> w2;   // [256][256] data-backward operand: row = f1, k = f2
+    // workspaces (grown on demand)
+    pe::i64 cap = 0;           // rows the workspaces hold (multiple of 128)
+    pe::DevBuf<uint16_t> x, xt, h1, h1t, h2t, dz2, dz2t, dz1t;
+    pe::DevBuf<float> y, ypart, dy, tgt, wpart, red;
+};
+
 struct pyci_op {
     pyci_ctx *ctx = nullptr;
     int kind = 0;
@@ -140,6 +174,10 @@ struct pyci_op {
     pe::DevBuf<double> diag;      // nrow
     // rows built in segments for non-symmetric update(): [row0, row1) with its ncol
     std::vector<std::pair<pe::i64, pe::i64>> segments;
+    // storage PYCI_STORAGE_DIRECT: matrix-free sigma; the CSR arrays above are
+    // filled only on demand (exports, element queries)
+    int storage = PYCI_STORAGE_CSR;
+    std::unique_ptr<pe::DirectOp> direct;
 };
 
 namespace pe {
@@ -258,5 +296,35 @@ void fill_all_dets_device(pyci_ctx *ctx, int kind, i64 nb, i64 nup, i64 ndn, u64
 // HCI / RDMs
 i64 add_hci_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, double eps);
 void rdms_device(pyci_wfn *wfn, const double *coeffs_host, double *rdm1, double *rdm2);
+
+// Heat-bath candidate pool (hci.hip): the new determinants passing eps_pool in
+// first-occurrence order, their heat-bath scores max_i |H_ji c_i|, and whether
+// plain add_hci(eps) would take each (nplain of them). dc: device coefficients.
+struct HciPool {
+    DevBuf<u64> dets;
+    DevBuf<double> hb;
+    DevBuf<int> plain;
+    i64 n = 0, nplain = 0;
+};
+void hci_pool_device(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double eps, double eps_pool, HciPool &P);
+
+// Matrix-free FullCI operator (direct.hip)
+bool direct_applicable(const pyci_ham *ham, const pyci_wfn *wfn, i64 nrow, i64 ncol, int symmetric);
+void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn);
+template <typename T>
+void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s);
+void direct_ensure_csr(pyci_op *op);
+
+// Learned selector (mlp.hip)
+void mlp_init(pyci_mlp *m, int nfeat, u64 seed);
+void mlp_set_params(pyci_mlp *m, const float *host);
+void mlp_get_params(pyci_mlp *m, float *host);
+void mlp_predict(pyci_mlp *m, int kind, i64 nbasis, i64 n, const u64 *dets, float *y);
+void mlp_predict_device(pyci_mlp *m, int kind, i64 nbasis, const u64 *d_dets, i64 n, float *d_y);
+float mlp_loss_grad(pyci_mlp *m, int kind, i64 nbasis, i64 n, const u64 *dets, const float *target, float *grad);
+float mlp_train(pyci_mlp *m, const pyci_wfn *wfn, const double *coeffs, double cmin, i64 steps, i64 batch, double lr,
+                u64 seed);
+i64 add_hci_mlp_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, double eps, pyci_mlp *m,
+                       double pool, i64 nadd, int mode);
 
 }  // namespace pe

@@ -157,10 +157,12 @@ static void spmm(const pyci_op *op, const VT *vals, int nvec, const XT *X, XT *Y
 }
 
 void sigma_f64(const pyci_op *op, int nvec, const double *X, double *Y, hipStream_t s) {
+    if (op->storage == PYCI_STORAGE_DIRECT) return direct_sigma<double>(op, nvec, X, Y, s);
     spmm<double, double>(op, op->vals.p, nvec, X, Y, s);
 }
 
 void sigma_f32(const pyci_op *op, int nvec, const float *X, float *Y, hipStream_t s) {
+    if (op->storage == PYCI_STORAGE_DIRECT) return direct_sigma<float>(op, nvec, X, Y, s);
     spmm<float, float>(op, op->vals32.p, nvec, X, Y, s);
 }
 
@@ -170,6 +172,7 @@ __global__ void k_to_f32(const double *a, float *b, i64 n) {
 }
 
 void ensure_vals32(pyci_op *op) {
+    if (op->storage == PYCI_STORAGE_DIRECT) return;  // the matrix-free operator has FP32 integrals of its own
     if (op->vals32.p && op->vals32.n >= (size_t)op->nnz) return;
     op->vals32.alloc((size_t)op->nnz);
     if (op->nnz > 0) {
@@ -189,7 +192,9 @@ __global__ void k_fill_random(double *x, i64 n, u64 seed) {
 
 // Times nrep launches of the sigma kernel with HIP events on the context stream.
 // Algorithmic bytes of one launch: every stored entry's column index and value,
-// the row pointers, one read of X and one write of Y.
+// the row pointers, one read of X and one write of Y. For the matrix-free
+// operator: one read of X, one write of Y, and one write and one read each of
+// the pair-indexed intermediates D and E (npair values per determinant and vector).
 void op_bench_sigma(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *avg_ms, double *bytes) {
     hipStream_t s = op->ctx->stream;
     const i64 nr = op->nrow, nc = op->ncol;
@@ -220,7 +225,10 @@ void op_bench_sigma(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *avg_ms, do
     float ms = 0.f;
     PE_HIP(hipEventElapsedTime(&ms, op->ctx->ev0, op->ctx->ev1));
     *avg_ms = ms / (float)nrep;
-    *bytes = (double)op->nnz * (4.0 + vb) + 8.0 * (double)(nr + 1) + (double)vb * (double)nvec * (double)(nc + nr);
+    if (op->storage == PYCI_STORAGE_DIRECT)
+        *bytes = (double)vb * (double)nvec * (double)nr * (2.0 + 4.0 * (double)op->direct->npair);
+    else
+        *bytes = (double)op->nnz * (4.0 + vb) + 8.0 * (double)(nr + 1) + (double)vb * (double)nvec * (double)(nc + nr);
 }
 
 }  // namespace pe

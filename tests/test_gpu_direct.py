@@ -1,0 +1,98 @@
+"""Matrix-free FullCI operator (storage "direct") against the CSR operator and
+the oracle: products, eigenvalues, exports materialised on demand, fallbacks."""
+import numpy as np
+import pytest
+
+import pyci_amd as pyci
+from pyci_amd.synthetic import seeded_integrals
+
+pytestmark = pytest.mark.gpu
+
+
+def fullci(nb, nu, nd, nchol=8, seed=3):
+    _, one, two = seeded_integrals(nb, nchol, seed)
+    ham = pyci.hamiltonian(0.0, one, two)
+    wfn = pyci.fullci_wfn(nb, nu, nd)
+    wfn.add_all_dets()
+    return ham, wfn, one, two
+
+
+@pytest.mark.parametrize("nb,nu,nd", [(4, 1, 1), (6, 3, 3), (7, 3, 2), (8, 4, 4), (9, 5, 2), (11, 5, 5)])
+def test_direct_matvec_matches_csr(nb, nu, nd):
+    ham, wfn, _, _ = fullci(nb, nu, nd, seed=nb)
+    a = pyci.sparse_op(ham, wfn, storage="csr")
+    b = pyci.sparse_op(ham, wfn, storage="direct")
+    assert a.storage == "csr" and b.storage == "direct"
+    assert b.size == a.size and b.shape == a.shape
+    rng = np.random.default_rng(nb)
+    for _ in range(2):
+        x = rng.normal(size=len(wfn))
+        ya, yb = a(x), b(x)
+        # the same Hamiltonian summed in another order (1e-12 relative)
+        assert np.max(np.abs(ya - yb)) <= 1e-12 * np.max(np.abs(ya))
+    np.testing.assert_array_equal(a.diagonal(), b.diagonal())
+
+
+def test_direct_default_and_exports_are_the_csr():
+    ham, wfn, _, _ = fullci(7, 3, 3, seed=4)
+    a = pyci.sparse_op(ham, wfn, storage="csr")
+    b = pyci.sparse_op(ham, wfn)  # auto -> direct for the full FullCI space
+    assert b.storage == "direct"
+    for i, j in [(0, 0), (5, 3), (100, 7), (len(wfn) - 1, len(wfn) - 2), (3, 9)]:
+        assert b.get_element(i, j) == a.get_element(i, j)
+    np.testing.assert_array_equal(b.indptr(), a.indptr())
+    np.testing.assert_array_equal(b.indices(), a.indices())
+    np.testing.assert_array_equal(b.data(), a.data())
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+@pytest.mark.parametrize("n", [1, 3])
+def test_direct_solve_matches_oracle(oracle, precision, n):
+    ham, wfn, one, two = fullci(8, 4, 3, seed=9)
+    dets = oracle.all_dets("fullci", 8, 4, 3)
+    ref = oracle.lowest_eigs(oracle.csr_build("fullci", 8, 4, 3, dets, one, two), n)[0]
+    op = pyci.sparse_op(ham, wfn)
+    es, cs = op.solve(n=n, tol=1e-10, precision=precision)
+    np.testing.assert_allclose(np.sort(es), ref, rtol=0, atol=1e-10)
+    # eigenvectors: residuals through the CSR operator
+    a = pyci.sparse_op(ham, wfn, storage="csr")
+    for e, c in zip(es, cs):
+        assert np.linalg.norm(a(c) - e * c) < 1e-7
+
+
+def test_direct_fallbacks_and_update(oracle):
+    _, one, two = seeded_integrals(6, 6, 2)
+    ham = pyci.hamiltonian(0.0, one, two)
+    sub = pyci.fullci_wfn(6, 2, 2)
+    pyci.add_excitations(sub, 0, 1)
+    assert pyci.sparse_op(ham, sub).storage == "csr"  # not the full space
+    with pytest.raises(ValueError):
+        pyci.sparse_op(ham, sub, storage="direct")
+    full = pyci.fullci_wfn(6, 2, 2)
+    full.add_all_dets()
+    with pytest.raises(ValueError):
+        pyci.sparse_op(ham, full, symmetric=False, storage="direct")
+    assert pyci.sparse_op(ham, full, nrow=10).storage == "csr"
+    with pytest.raises(ValueError):
+        pyci.sparse_op(ham, full, storage="blocked")
+    op = pyci.sparse_op(ham, full)
+    assert op.storage == "direct"
+    op.update(ham, full)  # unchanged space
+    assert op.storage == "direct"
+    # DOCI and GenCI use the CSR
+    d = pyci.doci_wfn(6, 3, 3)
+    d.add_all_dets()
+    assert pyci.sparse_op(ham, d).storage == "csr"
+
+
+def test_direct_be_fullci_energy():
+    """test_routines.py:37-54: be_ccpvdz FullCI -14.617409507 (nbasis 14, direct)."""
+    from tests.conftest import datafile
+
+    ham = pyci.hamiltonian(datafile("be_ccpvdz.fcidump"))
+    wfn = pyci.fullci_wfn(ham.nbasis, 2, 2)
+    wfn.add_all_dets()
+    op = pyci.sparse_op(ham, wfn)
+    assert op.storage == "direct"
+    es, _ = op.solve(n=1, tol=1e-9)
+    assert abs(es[0] - (-14.617409507)) < 1e-8
