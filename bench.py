@@ -6,18 +6,24 @@ determinants; 1,553,018,544 stored nonzeros), seeded integrals (48 Cholesky-like
 vectors, seed 1001), ecore 0, nroot 1.
 
 A step is one pass of the hot path with integrals and determinants already in
-HBM: CSR build (pyci_op_create) + Davidson solve (pyci_op_solve) until the
-energy is within 1e-8 Eh of the FP64 reference energy. `value` is the mean
-time-to-solution of the timed steps (seconds, lower is better).
+HBM: sparse_op (pyci_op_create_ex, storage --storage, default "auto": the
+matrix-free FullCI operator for this full space) + Davidson solve
+(pyci_op_solve, precision --precision) until the energy is within 1e-8 Eh of
+the FP64 reference energy. `value` is the mean time-to-solution of the timed
+steps (seconds, lower is better). The plain FP64 GPU path (CSR build + FP64
+solve) is timed the same way beside it ("baseline_csr_f64").
 
-The FP64 reference energy of this workload comes from an untimed FP64 solve at
-tol 1e-12 of the same operator (the CSR builder and Davidson are parity-checked
-against the CPU oracle on smaller systems by the test suite).
+The FP64 reference energy comes from an untimed FP64 solve at tol 1e-12 of the
+CSR operator (the CSR builder and both operators are parity-checked against the
+CPU oracle by the test suite).
 
-roofline: the dominant kernel is the sigma SpMV (Y = H x) over the stored CSR
-(int32 columns + values, int64 row pointers). Its algorithmic bytes per launch
-are 12*nnz + 8*(nrow+1) + 8*(ncol+nrow) for FP64 values; its average duration
-is measured with HIP events on the engine's stream (pyci_op_bench_sigma).
+roofline: the dominant kernel of one sigma (Y = H x) of the timed operator.
+For the CSR it is the SpMV (algorithmic bytes 12*nnz + 8*(nrow+1) +
+8*(ncol+nrow) in FP64); for the matrix-free operator the slowest of its three
+kernels (gather pass, pair-integral product, scatter pass; bytes and flops per
+launch in pyci_op_bench_sigma_parts). Durations are HIP events on the engine's
+stream; "bound" is "mfma" when the kernel's flops/byte exceed the ridge point
+of its precision (peak FLOP/s / 8 TB/s), else "hbm".
 
 cpu_baseline: the CPU oracle (a C restatement of the reference's serial
 SparseOp::update + perform_op) timed on the host cores on a bounded sample of
@@ -39,9 +45,12 @@ sys.path.insert(0, ROOT)
 from pyci_amd.synthetic import config_integrals  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
+MFMA_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 WORKLOADS = {
     "fci12": dict(name="fci12", nbasis=12, nup=6, ndn=6, integrals="fci12", nroot=1),
     "fci8": dict(name="fci8", nbasis=8, nup=4, ndn=4, integrals="fci8", nroot=1),
+    # nbasis 14: the CSR (39.8 G nonzeros) does not fit in HBM; matrix-free only
+    "fci14": dict(name="fci14", nbasis=14, nup=7, ndn=7, integrals="fci14", nroot=8, csr=False),
 }
 
 
@@ -64,6 +73,10 @@ def engine():
     L.pyci_wfn_create.argtypes = [vp, ctypes.c_int, i64, i64, i64, i64, ctypes.c_void_p, ctypes.POINTER(vp)]
     L.pyci_fill_all_dets.argtypes = [vp, ctypes.c_int, i64, i64, i64, ctypes.c_void_p]
     L.pyci_op_create.argtypes = [vp, vp, i64, i64, ctypes.c_int, ctypes.POINTER(vp)]
+    L.pyci_op_create_ex.argtypes = [vp, vp, i64, i64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+    L.pyci_op_storage.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.pyci_op_bench_sigma_parts.argtypes = [vp, i64, i64, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_float), dp, dp]
     L.pyci_op_destroy.argtypes = [vp]
     L.pyci_op_info.argtypes = [vp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                ctypes.c_void_p, ctypes.c_void_p]
@@ -81,10 +94,10 @@ def dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-def measured_traffic(name):
-    """HBM bytes per sigma launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary (profiles/sigma_<workload>_hbm_traffic.md), or None."""
-    path = os.path.join(ROOT, "profiles", f"sigma_{name}_hbm_traffic.md")
+def measured_traffic(fname):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary under profiles/, or None."""
+    path = os.path.join(ROOT, "profiles", fname)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -163,6 +176,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="fci12", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="f64", choices=["f64", "mixed"])
+    ap.add_argument("--storage", default="auto", choices=["auto", "csr", "direct"])
+    ap.add_argument("--nroot", type=int, default=0, help="roots (default: the workload's)")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -194,12 +209,30 @@ def main():
        "wfn_create")
     log(f"workload {wl['name']}: {nd} determinants resident in HBM (colex fill {t_fill*1e3:.1f} ms)")
     prec = 0 if args.precision == "f64" else 1
-    nroot = wl["nroot"]
+    nroot = args.nroot or wl["nroot"]
+    has_csr = wl.get("csr", True)
 
-    def build():
+    storage_code = {"csr": 0, "direct": 1, "auto": 2}
+
+    def build(storage=args.storage):
         op = ctypes.c_void_p()
-        ok(L, L.pyci_op_create(ham, wfn, -1, -1, 1, ctypes.byref(op)), "op_create")
+        ok(L, L.pyci_op_create_ex(ham, wfn, -1, -1, 1, storage_code[storage], ctypes.byref(op)), "op_create")
         return op
+
+    def storage_of(op):
+        st = ctypes.c_int()
+        ok(L, L.pyci_op_storage(op, ctypes.byref(st)), "op_storage")
+        return "direct" if st.value == 1 else "csr"
+
+    def parts(op, nvec, nrep, fp32):
+        n = ctypes.c_int()
+        ms = (ctypes.c_float * 4)()
+        by = (ctypes.c_double * 4)()
+        fl = (ctypes.c_double * 4)()
+        ok(L, L.pyci_op_bench_sigma_parts(op, nvec, nrep, fp32, ctypes.byref(n), ms,
+                                          ctypes.cast(by, ctypes.POINTER(ctypes.c_double)),
+                                          ctypes.cast(fl, ctypes.POINTER(ctypes.c_double))), "bench_sigma_parts")
+        return [dict(ms=ms[i], bytes=by[i], flops=fl[i]) for i in range(n.value)]
 
     def info(op):
         v = [ctypes.c_int64() for _ in range(4)]
@@ -217,53 +250,69 @@ def main():
     if args.sigma_only:
         op = build()
         nrow, ncol, size_ref, nnz = info(op)
-        ms, nbytes = ctypes.c_float(), ctypes.c_double()
-        ok(L, L.pyci_op_bench_sigma(op, 1, 4, 0, ctypes.byref(ms), ctypes.byref(nbytes)), "bench_sigma")
+        pr = parts(op, 1, 4, 1 if args.precision == "mixed" else 0)
         copy_bytes = 4 << 30
         t_copy = device_copy(copy_bytes)
-        print(json.dumps({"sigma_avg_ms": ms.value, "sigma_bytes": nbytes.value, "nnz": nnz,
-                          "copy_bytes": copy_bytes, "copy_ms": t_copy * 1e3}), flush=True)
+        print(json.dumps({"storage": storage_of(op), "parts": pr, "nnz": nnz, "copy_bytes": copy_bytes,
+                          "copy_ms": t_copy * 1e3}), flush=True)
         return
-    # FP64 reference energy (untimed, tight tolerance)
-    op = build()
+    # FP64 reference energies (untimed, tight tolerance): from the CSR operator,
+    # or from the matrix-free FP64 operator where the CSR cannot be stored
+    op = build("csr" if has_csr else "direct")
     nrow, ncol, size_ref, nnz = info(op)
-    e_ref, st_ref = solve(op, 0, 1e-12)
-    log(f"reference FP64 energy {e_ref[0]:.12f} ({st_ref.iterations} iterations); stored nnz {nnz}, "
-        f"reference-format nnz {size_ref}")
-    # sigma kernel on the same operator (HIP events on the engine stream)
-    ms, nbytes = ctypes.c_float(), ctypes.c_double()
-    ok(L, L.pyci_op_bench_sigma(op, 1, 20, 0, ctypes.byref(ms), ctypes.byref(nbytes)), "bench_sigma")
-    sig = dict(avg_ms=ms.value, bytes=nbytes.value)
-    ok(L, L.pyci_op_bench_sigma(op, 1, 20, 1, ctypes.byref(ms), ctypes.byref(nbytes)), "bench_sigma f32")
-    sig32 = dict(avg_ms=ms.value, bytes=nbytes.value)
-    ok(L, L.pyci_op_bench_sigma(op, 8, 10, 0, ctypes.byref(ms), ctypes.byref(nbytes)), "bench_sigma nv8")
-    sig8 = dict(avg_ms=ms.value, bytes=nbytes.value)
+    e_ref, st_ref = solve(op, 0, 1e-12 if has_csr else 1e-10)
+    log(f"reference FP64 energies {np.array2string(e_ref, precision=10)} ({st_ref.iterations} iterations); "
+        f"stored nnz {nnz}, reference-format nnz {size_ref}")
+    csr_parts = None
+    if has_csr:
+        # CSR sigma (HIP events on the engine stream): the plain path's kernel
+        csr_parts = {"f64": parts(op, 1, 20, 0)[0], "f32": parts(op, 1, 20, 1)[0],
+                     "f64_nvec8": parts(op, 8, 10, 0)[0]}
+        for k, v in csr_parts.items():
+            log(f"CSR sigma {k}: {v['ms']:.3f} ms, {v['bytes'] / v['ms'] / 1e6:.0f} GB/s")
     L.pyci_op_destroy(op)
-    log(f"sigma f64: {sig['avg_ms']:.3f} ms, {sig['bytes']/sig['avg_ms']/1e6:.0f} GB/s; "
-        f"f32: {sig32['avg_ms']:.3f} ms, {sig32['bytes']/sig32['avg_ms']/1e6:.0f} GB/s; "
-        f"f64 nvec=8: {sig8['avg_ms']:.3f} ms")
 
-    def step():
+    def step(storage, p):
         ok(L, L.pyci_ctx_synchronize(ctx), "sync")
         t0 = time.perf_counter()
-        o = build()
+        o = build(storage)
         ok(L, L.pyci_ctx_synchronize(ctx), "sync")
         t1 = time.perf_counter()
-        e, st = solve(o, prec, args.tol)
+        e, st = solve(o, p, args.tol)
         ok(L, L.pyci_ctx_synchronize(ctx), "sync")
         t2 = time.perf_counter()
         L.pyci_op_destroy(o)
         return t1 - t0, t2 - t1, e, st
 
+    base = None
+    if has_csr:
+        # plain FP64 GPU path: CSR build + FP64 solve (one warm-up, then timed)
+        step("csr", 0)
+        base = [step("csr", 0) for _ in range(2)]
+        t_base = float(np.mean([b[0] + b[1] for b in base]))
+        log(f"baseline CSR f64: {t_base*1e3:.1f} ms (build {np.mean([b[0] for b in base])*1e3:.1f} ms, "
+            f"{base[0][3].iterations} iterations), |dE| {np.max(np.abs(base[0][2] - e_ref)):.2e}")
+    # the timed operator's sigma kernels (one vector per root block column)
+    op = build()
+    storage = storage_of(op)
+    fp32 = 1 if args.precision == "mixed" else 0
+    tparts = parts(op, nroot, 20 if nroot == 1 else 3, fp32)
+    L.pyci_op_destroy(op)
+    names = ["k_spmv<4> (CSR sigma)"] if storage == "csr" else [
+        "k_mf_d (gather pass)", "k_mf_gemm (pair-integral product, MFMA)", "k_mf_sigma (scatter pass)"]
+    for nm, v in zip(names, tparts):
+        log(f"{storage} sigma part {nm}: {v['ms']:.3f} ms, {v['bytes'] / v['ms'] / 1e6:.0f} GB/s, "
+            f"{v['flops'] / v['ms'] / 1e9:.1f} TFLOP/s")
+
     for w in range(args.warmup):
-        tb, ts, e, st = step()
+        tb, ts, e, st = step(args.storage, prec)
         log(f"warmup {w}: build {tb*1e3:.1f} ms, solve {ts*1e3:.1f} ms")
     ok(L, L.pyci_ctx_synchronize(ctx), "sync")
     res = []
     t_all0 = time.perf_counter()
     for k in range(args.steps):
-        tb, ts, e, st = step()
-        de = abs(e[0] - e_ref[0])
+        tb, ts, e, st = step(args.storage, prec)
+        de = float(np.max(np.abs(e - e_ref)))
         res.append((tb, ts, de, st.iterations, st.sigma_calls))
         log(f"step {k}: build {tb*1e3:.1f} ms, solve {ts*1e3:.1f} ms, |dE| {de:.2e}, {st.iterations} iterations")
     ok(L, L.pyci_ctx_synchronize(ctx), "sync")
@@ -273,12 +322,25 @@ def main():
     ts = float(np.mean([r[1] for r in res]))
     max_de = max(r[2] for r in res)
     sigma_calls = int(np.mean([r[4] for r in res]))
-    achieved = sig["bytes"] / (sig["avg_ms"] * 1e-3) / 1e9
-    traffic = measured_traffic(wl["name"])
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                "kernel": "k_spmv<4,double,double> (sigma, FP64 values)", "avg_ms": round(sig["avg_ms"], 4),
-                "bytes_per_launch": sig["bytes"]}
+    # roofline of the dominant kernel of the timed operator's sigma
+    dom = int(np.argmax([v["ms"] for v in tparts]))
+    dk = tparts[dom]
+    pk = "f32" if fp32 else "f64"
+    ridge = MFMA_PEAK_TFLOPS[pk] * 1e12 / (HBM_PEAK_GBPS * 1e9)
+    kname = names[dom]
+    slug = "sigma_csr" if storage == "csr" else ["mf_gather", "mf_product", "mf_scatter"][dom]
+    traffic = measured_traffic(f"{slug}_{wl['name']}_{pk}_hbm_traffic.md")
+    if dk["flops"] / dk["bytes"] > ridge:
+        achieved = dk["flops"] / (dk["ms"] * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": MFMA_PEAK_TFLOPS[pk], "unit": "TFLOP/s",
+                    "frac": round(achieved / MFMA_PEAK_TFLOPS[pk], 4)}
+    else:
+        achieved = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBPS, 4)}
+    roofline.update({"traffic": traffic, "kernel": f"{kname}, {pk}", "avg_ms": round(dk["ms"], 4),
+                     "bytes_per_launch": dk["bytes"], "flops_per_launch": dk["flops"]})
+
     out = {
         "metric": "time-to-|dE|<1e-8 Eh vs FP64 oracle; sigma HBM GB/s; dets/s built",
         "value": round(tts, 6),
@@ -292,29 +354,39 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec == 0 else "f32-sigma+f64",
         "data": "synthetic (seeded integrals)",
-        "config": {"workload": f"FullCI nbasis={nb} {nu}a+{nd_}b seeded integrals (48 vectors, seed 1001), nroot=1",
+        "config": {"workload": f"FullCI nbasis={nb} {nu}a+{nd_}b seeded integrals ({wl['integrals']}), nroot={nroot}",
                    "ndet": nd, "nnz_stored": nnz, "nnz_reference_format": size_ref, "parallelism": "single GPU",
                    "precision": args.precision, "tol": args.tol},
+        "storage": storage,
         "build_ms": round(tb * 1e3, 3),
         "solve_ms": round(ts * 1e3, 3),
-        "dets_per_s_built": round(nd / tb, 1),
-        "nnz_per_s_built": round(nnz / tb, 1),
-        "energy": float(e_ref[0]),
+        "energy": [float(x) for x in e_ref],
         "max_abs_dE": max_de,
         "iterations": int(np.mean([r[3] for r in res])),
         "sigma_calls": sigma_calls,
-        "sigma_gbps": round(achieved, 1),
-        "sigma_f32_gbps": round(sig32["bytes"] / (sig32["avg_ms"] * 1e-3) / 1e9, 1),
-        "sigma_nvec8_ms": round(sig8["avg_ms"], 4),
+        "sigma_parts_ms": {nm: round(v["ms"], 4) for nm, v in zip(names, tparts)},
         "roofline": roofline,
     }
-    if not args.no_cpu:
+    if has_csr:
+        tbase_build = float(np.mean([b[0] for b in base]))
+        sig = csr_parts["f64"]
+        out.update({
+            "baseline_csr_f64": {"tts_s": round(t_base, 6), "build_ms": round(tbase_build * 1e3, 3),
+                                 "iterations": int(base[0][3].iterations), "speedup": round(t_base / tts, 2)},
+            "dets_per_s_built": round(nd / tbase_build, 1),
+            "nnz_per_s_built": round(nnz / tbase_build, 1),
+            "sigma_gbps": round(sig["bytes"] / (sig["ms"] * 1e-3) / 1e9, 1),
+            "sigma_f32_gbps": round(csr_parts["f32"]["bytes"] / (csr_parts["f32"]["ms"] * 1e-3) / 1e9, 1),
+            "sigma_nvec8_ms": round(csr_parts["f64_nvec8"]["ms"], 4),
+        })
+    if not args.no_cpu and has_csr:
         log("cpu baseline: oracle on a bounded row sample ...")
-        cb = cpu_baseline(wl, one, two, dets, sigma_calls, args.cpu_seconds)
+        cpu_sigma = int(base[0][3].sigma_calls)  # the plain path's product count
+        cb = cpu_baseline(wl, one, two, dets, cpu_sigma, args.cpu_seconds)
         frac = size_ref / max(cb["nnz_sample"], 1)  # full lower nnz / sample lower nnz
         t_build_cpu = cb["t_index"] + cb["t_row"] * nd
         t_spmv_cpu = cb["t_spmv_sample"] * frac
-        t_cpu = t_build_cpu + sigma_calls * t_spmv_cpu
+        t_cpu = t_build_cpu + cpu_sigma * t_spmv_cpu
         out["cpu_baseline"] = {
             "value": round(t_cpu, 3),
             "unit": "s",
@@ -323,7 +395,7 @@ def main():
             "sample": (f"oracle CSR build of {cb['rows']} middle rows of {nd} ({cb['t_row']*1e6:.1f} us/row, "
                        f"index {cb['t_index']:.2f} s) and serial lower-triangle SpMV over them "
                        f"({cb['t_spmv_sample']*1e3:.1f} ms for {cb['nnz_sample']} nnz), extrapolated to the full "
-                       f"build + {sigma_calls} sigma products"),
+                       f"build + {cpu_sigma} sigma products"),
             "dets_per_s_built": round(1.0 / cb["t_row"], 1),
         }
         log(f"cpu baseline estimate {t_cpu:.1f} s")

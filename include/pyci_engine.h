@@ -144,6 +144,12 @@ int pyci_op_matvec_device(pyci_op *op, int64_t nvec, const double *x_dev, double
  * random X (nvec columns); returns the average kernel time from HIP events and the
  * algorithmic bytes of one launch. fp32 != 0 times the FP32-value kernel. */
 int pyci_op_bench_sigma(pyci_op *op, int64_t nvec, int64_t nrep, int fp32, float *avg_ms, double *bytes);
+/* Per-kernel view of the same measurement: *nparts kernels per sigma (1 for the
+ * CSR; 3 for the matrix-free operator: gather pass, pair-integral product, scatter
+ * pass), each with its average duration (HIP events between the kernels of each
+ * of nrep sigmas), algorithmic bytes and flops per launch. Arrays hold >= 4. */
+int pyci_op_bench_sigma_parts(pyci_op *op, int64_t nvec, int64_t nrep, int fp32, int *nparts, float *avg_ms,
+                              double *bytes, double *flops);
 /* solve(n, c0, ncv, maxiter, tol) (sparseop.cpp:114-146,161-173): lowest n
  * eigenpairs by block Davidson on the GPU. evals (n, ecore added) and evecs
  * (n, nrow) are returned in the reference order: descending eigenvalue when

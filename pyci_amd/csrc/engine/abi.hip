@@ -473,6 +473,23 @@ int pyci_op_bench_sigma(pyci_op *op, int64_t nvec, int64_t nrep, int fp32, float
     });
 }
 
+int pyci_op_bench_sigma_parts(pyci_op *op, int64_t nvec, int64_t nrep, int fp32, int *nparts, float *avg_ms,
+                              double *bytes, double *flops) {
+    return guarded([&] {
+        check_matvec_shape(op);
+        PE_REQUIRE(nvec >= 1 && nrep >= 1, PYCI_EINVAL, "nvec and nrep must be >= 1");
+        set_device(op->ctx);
+        if (op->storage == PYCI_STORAGE_DIRECT) {
+            *nparts = 3;
+            direct_bench_parts(op, nvec, nrep, fp32, avg_ms, bytes, flops);
+        } else {
+            *nparts = 1;
+            op_bench_sigma(op, nvec, nrep, fp32, avg_ms, bytes);
+            flops[0] = 2.0 * (double)op->nnz * (double)nvec;
+        }
+    });
+}
+
 int pyci_op_solve(pyci_op *op, int64_t n, const double *c0, int64_t ncv, int64_t maxiter, double tol, int precision,
                   double *evals, double *evecs, pyci_solve_stats *stats) {
     return guarded([&] {

@@ -79,7 +79,8 @@ __global__ void __launch_bounds__(256) k_mf_d(i64 nd, i64 NB, int npair, int nea
 // wave owns 32 columns (two 16-column tiles) and all MT row tiles; every column
 // is read completely before the wave writes it. MFMA 16x16x4: lane l supplies
 // A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator r of lane l is
-// D[4 (l / 16) + r][l % 16].
+// D[4 (l / 16) + r][l % 16] for the FP32 instruction and D[l / 16 + 4 r][l % 16]
+// for the FP64 one (lane layouts checked on the device).
 template <int MT, typename T>
 __global__ void __launch_bounds__(256) k_mf_gemm(i64 ncol, const T *__restrict__ V, const T *__restrict__ kv,
                                                  const T *__restrict__ X, T *W, i64 ldw) {
@@ -118,7 +119,7 @@ __global__ void __launch_bounds__(256) k_mf_gemm(i64 ncol, const T *__restrict__
         for (int m = 0; m < MT; ++m)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int p = 16 * m + 4 * lk + r;
+                const int p = (sizeof(T) == 8) ? 16 * m + lk + 4 * r : 16 * m + 4 * lk + r;
                 W[(i64)p * ldw + c] = (T)0.5 * acc[m][t][r] + kv[p] * x;
             }
     }
@@ -157,6 +158,16 @@ __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int n
 #pragma unroll
     for (int v = 0; v < NV; ++v)
         if (v < nvec) y[v] = acc[v];
+}
+
+__global__ void k_direct_fill(double *x, i64 n, u64 seed) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        x[i] = (double)(mix64(seed + (u64)i) >> 11) * 0x1p-53 - 0.5;
+}
+
+__global__ void k_direct_to_f32(const double *a, float *b, i64 n) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
 }
 
 #define GIX(p, q, r, s) ((((p) * n + (q)) * n + (r)) * n + (s))
@@ -373,12 +384,15 @@ static void launch_gemm(i64 ncol, const T *V, const T *kv, const T *X, T *W, i64
     PE_LAUNCH_CHECK();
 }
 
+// ev (optional): 4 events recorded before, between and after the three passes.
 template <int NV, typename T>
 static void launch_passes(const DirectOp *D, i64 nd, int nvec, const T *X, T *W, i64 ldw, T *Y, const T *V,
-                          const T *kv, hipStream_t s) {
+                          const T *kv, hipStream_t s, hipEvent_t *ev) {
     const unsigned g = grid_for(nd, 256);
+    if (ev) PE_HIP(hipEventRecord(ev[0], s));
     k_mf_d<NV, T><<<g, 256, 0, s>>>(nd, D->nbs, D->npair, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, X, W, ldw);
     PE_LAUNCH_CHECK();
+    if (ev) PE_HIP(hipEventRecord(ev[1], s));
     const i64 ncol = nd * nvec;
     switch (D->kpad / 16) {
         case 1: launch_gemm<1, T>(ncol, V, kv, X, W, ldw, s); break;
@@ -391,12 +405,14 @@ static void launch_passes(const DirectOp *D, i64 nd, int nvec, const T *X, T *W,
         case 8: launch_gemm<8, T>(ncol, V, kv, X, W, ldw, s); break;
         default: throw EngineError(PYCI_EUNSUPPORTED, "pair dimension too large for the matrix-free operator");
     }
+    if (ev) PE_HIP(hipEventRecord(ev[2], s));
     k_mf_sigma<NV, T><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, W, ldw, Y);
     PE_LAUNCH_CHECK();
+    if (ev) PE_HIP(hipEventRecord(ev[3], s));
 }
 
 template <typename T>
-void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s) {
+void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, hipEvent_t *ev) {
     DirectOp *D = op->direct.get();
     PE_REQUIRE(nvec >= 1 && nvec <= 16, PYCI_EUNSUPPORTED, "the matrix-free operator takes 1..16 vectors");
     const i64 nd = op->nrow;
@@ -405,15 +421,66 @@ void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s) 
     const T *V = std::is_same<T, double>::value ? (const T *)D->V64.p : (const T *)D->V32.p;
     const T *kv = std::is_same<T, double>::value ? (const T *)D->k64.p : (const T *)D->k32.p;
     const i64 ldw_used = std::is_same<T, double>::value ? D->ldw64 : D->ldw32;
-    if (nvec == 1) launch_passes<1, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s);
-    else if (nvec == 2) launch_passes<2, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s);
-    else if (nvec <= 4) launch_passes<4, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s);
-    else if (nvec <= 8) launch_passes<8, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s);
-    else launch_passes<16, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s);
+    if (nvec == 1) launch_passes<1, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s, ev);
+    else if (nvec == 2) launch_passes<2, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s, ev);
+    else if (nvec <= 4) launch_passes<4, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s, ev);
+    else if (nvec <= 8) launch_passes<8, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s, ev);
+    else launch_passes<16, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s, ev);
 }
 
-template void direct_sigma<double>(const pyci_op *, int, const double *, double *, hipStream_t);
-template void direct_sigma<float>(const pyci_op *, int, const float *, float *, hipStream_t);
+template void direct_sigma<double>(const pyci_op *, int, const double *, double *, hipStream_t, hipEvent_t *);
+template void direct_sigma<float>(const pyci_op *, int, const float *, float *, hipStream_t, hipEvent_t *);
+
+// Per-pass timing of nrep matrix-free sigmas on random X (see pyci_op_bench_sigma_parts).
+// Algorithmic bytes and flops per launch, with N = nd * nvec columns and T the value size:
+//   gather pass:  reads X (T N), writes D (T npair N)
+//   product:      reads D and X, writes E (T (2 npair + 1) N); 2 npair^2 N flops
+//   scatter pass: reads E (T npair N), writes Y (T N)
+void direct_bench_parts(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *ms, double *bytes, double *flops) {
+    DirectOp *D = op->direct.get();
+    hipStream_t s = op->ctx->stream;
+    const i64 nd = op->nrow, N = nd * nvec;
+    DevBuf<double> x((size_t)N), y((size_t)N);
+    DevBuf<float> x32, y32;
+    k_direct_fill<<<1024, 256, 0, s>>>(x.p, N, 12345);
+    PE_LAUNCH_CHECK();
+    if (fp32) {
+        x32.alloc((size_t)N);
+        y32.alloc((size_t)N);
+        k_direct_to_f32<<<1024, 256, 0, s>>>(x.p, x32.p, N);
+        PE_LAUNCH_CHECK();
+    }
+    std::vector<hipEvent_t> ev((size_t)(4 * nrep));
+    for (auto &e : ev) PE_HIP(hipEventCreate(&e));
+    auto run = [&](hipEvent_t *e) {
+        if (fp32) direct_sigma<float>(op, (int)nvec, x32.p, y32.p, s, e);
+        else direct_sigma<double>(op, (int)nvec, x.p, y.p, s, e);
+    };
+    try {
+        run(nullptr);  // warm-up (workspace allocation)
+        for (i64 r = 0; r < nrep; ++r) run(&ev[(size_t)(4 * r)]);
+        PE_HIP(hipEventSynchronize(ev.back()));
+        double acc[3] = {0, 0, 0};
+        for (i64 r = 0; r < nrep; ++r)
+            for (int p = 0; p < 3; ++p) {
+                float t = 0.f;
+                PE_HIP(hipEventElapsedTime(&t, ev[(size_t)(4 * r + p)], ev[(size_t)(4 * r + p + 1)]));
+                acc[p] += t;
+            }
+        const double T = fp32 ? 4.0 : 8.0, P = (double)D->npair, NN = (double)N;
+        for (int p = 0; p < 3; ++p) ms[p] = (float)(acc[p] / (double)nrep);
+        bytes[0] = T * NN * (1.0 + P);
+        bytes[1] = T * NN * (2.0 * P + 1.0);
+        bytes[2] = T * NN * (P + 1.0);
+        flops[0] = 2.0 * (double)(D->nea + D->neb) * NN;
+        flops[1] = 2.0 * P * P * NN;
+        flops[2] = 2.0 * (double)(D->nea + D->neb) * NN;
+    } catch (...) {
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+}
 
 // Materialises the CSR of a matrix-free operator (for exports and element
 // queries), from a regenerated full-space wave function and the stored integrals.

@@ -312,7 +312,8 @@ void hci_pool_device(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double eps,
 bool direct_applicable(const pyci_ham *ham, const pyci_wfn *wfn, i64 nrow, i64 ncol, int symmetric);
 void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn);
 template <typename T>
-void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s);
+void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, hipEvent_t *ev = nullptr);
+void direct_bench_parts(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *ms, double *bytes, double *flops);
 void direct_ensure_csr(pyci_op *op);
 
 // Learned selector (mlp.hip)

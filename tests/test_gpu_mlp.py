@@ -143,16 +143,18 @@ def _heat_bath_scores(oracle, nb, ne, dets, cand, one, two, c):
 
 @pytest.mark.parametrize("score_mode", [0, 1])
 def test_selection_step_matches_oracle(oracle, score_mode):
-    n, m = 6, 2
+    n, m = 7, 2
     one, two = genci_problem(n)
     ham = pyci.hamiltonian(0.0, one, two)
     wfn, _ = hf_sd_wfn(n, m)
     rng = np.random.default_rng(21)
-    c = rng.normal(size=len(wfn)) * np.exp(-rng.random(len(wfn)) * 5)
+    # amplitudes spread over orders of magnitude: the pool (eps/4) is about twice
+    # the plain selection (eps)
+    c = rng.normal(size=len(wfn)) * np.exp(-rng.random(len(wfn)) * 5) * 1e-2
     sel = pyci.mlp_selector(2 * n, seed=4)
     sel.params = perturbed_params(sel, 5)
     dets = wfn.to_det_array().reshape(-1).copy()
-    eps, pool = 2e-3, 4.0
+    eps, pool = 5e-4, 4.0
     plain = oracle.add_hci("genci", 2 * n, 2 * m, 0, dets, one, two, c, eps)
     cand = oracle.add_hci("genci", 2 * n, 2 * m, 0, dets, one, two, c, eps / pool)
     assert plain.size > 0 and cand.size > plain.size
