@@ -144,6 +144,19 @@ __global__ void k_scale(double *x, i64 n, double s) {
     for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) x[i] *= s;
 }
 
+// Normalises x by the norm whose square is nn[1]; a vector whose norm fell below
+// 1e-10 of its norm before orthogonalisation (square in nn[0]) is numerically
+// dependent on the basis and is zeroed instead (the host drops it later).
+__device__ inline double checked_scale(const double *nn) {
+    const double n0 = sqrt(fmax(nn[0], 0.0)), n1 = sqrt(fmax(nn[1], 0.0));
+    return (n1 > 1e-10 * fmax(n0, 1e-300) && n1 > 1e-300) ? 1.0 / n1 : 0.0;
+}
+
+__global__ void k_scale_checked(double *x, i64 n, const double *nn) {
+    const double s = checked_scale(nn);
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) x[i] *= s;
+}
+
 // col-major (nd x k, ld) <-> row-major (nd x k), with optional precision change
 template <typename S, typename D>
 __global__ void k_col2row(i64 nd, int k, const S *src, i64 ld, D *dst) {
@@ -217,6 +230,7 @@ struct Davidson {
     i64 nd;
     int k, mmax;
     DevBuf<double> V, W, X, WX, T, rowbuf, rowout, partial, small;
+    DevBuf<double> RV, RW, dnorm;  // thick-restart scratch (nd x 2k each); per-vector norm pairs
     DevBuf<float> row32, out32;
     std::vector<double> G;  // mmax x mmax row-major
     int m = 0;
@@ -234,6 +248,7 @@ struct Davidson {
         nblk = (int)std::min<i64>(512, (nd + DT - 1) / DT);
         partial.alloc((size_t)nblk * (size_t)std::max(mmax * k, 64) + 64);
         small.alloc((size_t)(mmax * mmax + 4 * mmax + 64));
+        dnorm.alloc(64);
         G.assign((size_t)mmax * mmax, 0.0);
     }
 
@@ -322,6 +337,70 @@ struct Davidson {
         return true;
     }
 
+    // Appends column V[:, m] (already copied in) orthogonalised twice against
+    // V[:, 0:m] and normalised, with no host round trip: the norms before and
+    // after go to dnorm[2 slot], dnorm[2 slot + 1]; a dependent vector is zeroed.
+    void orthonormalize_device(int slot) {
+        double *col = V.p + (i64)m * nd;
+        gram(col, 1, col, 1, dnorm.p + 2 * slot);
+        for (int pass = 0; pass < 2 && m > 0; ++pass) {
+            double *C = small.p + (size_t)mmax * mmax + 2 * mmax;
+            gram_cm(V.p, m, col, 1, C);
+            k_gemm_update<<<g1(), 256, 0, s>>>(nd, V.p, nd, m, C, 1, -1.0, col, nd);
+            PE_LAUNCH_CHECK();
+        }
+        gram(col, 1, col, 1, dnorm.p + 2 * slot + 1);
+        k_scale_checked<<<g1(), 256, 0, s>>>(col, nd, dnorm.p + 2 * slot);
+        PE_LAUNCH_CHECK();
+        ++m;
+    }
+
+    // Fills G for the cnt new columns m0..m0+cnt-1 (one device-to-host copy that
+    // also brings the norm pairs of orthonormalize_device) and removes the columns
+    // that were zeroed as dependent. Returns the number of columns kept.
+    int update_G_checked(int m0, int cnt) {
+        gram(V.p, m0 + cnt, W.p + (i64)m0 * nd, cnt, small.p);
+        std::vector<double> h((size_t)(m0 + cnt) * cnt), nn((size_t)(2 * cnt));
+        PE_HIP(hipMemcpyAsync(h.data(), small.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+        PE_HIP(hipMemcpyAsync(nn.data(), dnorm.p, sizeof(double) * nn.size(), hipMemcpyDeviceToHost, s));
+        PE_HIP(hipStreamSynchronize(s));
+        std::vector<int> keep;
+        for (int j = 0; j < cnt; ++j) {
+            const double n0 = std::sqrt(std::max(nn[2 * j], 0.0)), n1 = std::sqrt(std::max(nn[2 * j + 1], 0.0));
+            if (n1 > 1e-10 * std::max(n0, 1e-300) && n1 > 1e-300) keep.push_back(j);
+        }
+        // compact kept columns to m0, m0+1, ... (device copies; zeroed columns dropped)
+        int dst = m0;
+        for (int j : keep) {
+            if (m0 + j != dst) {
+                PE_HIP(hipMemcpyAsync(V.p + (i64)dst * nd, V.p + (i64)(m0 + j) * nd, sizeof(double) * nd,
+                                      hipMemcpyDeviceToDevice, s));
+                PE_HIP(hipMemcpyAsync(W.p + (i64)dst * nd, W.p + (i64)(m0 + j) * nd, sizeof(double) * nd,
+                                      hipMemcpyDeviceToDevice, s));
+            }
+            ++dst;
+        }
+        // G rows/columns: old basis rows i < m0 and the kept new ones
+        std::vector<int> rows;
+        for (int i = 0; i < m0; ++i) rows.push_back(i);
+        for (int j : keep) rows.push_back(m0 + j);
+        for (int a = 0; a < (int)keep.size(); ++a) {
+            const int j = keep[(size_t)a];
+            for (int r = 0; r < (int)rows.size(); ++r) {
+                const double x = h[(size_t)rows[(size_t)r] * cnt + j];
+                G[(size_t)r * mmax + m0 + a] = x;
+                G[(size_t)(m0 + a) * mmax + r] = x;
+            }
+        }
+        for (int a = 0; a < (int)keep.size(); ++a)
+            for (int b = a + 1; b < (int)keep.size(); ++b) {
+                double x = 0.5 * (G[(size_t)(m0 + a) * mmax + m0 + b] + G[(size_t)(m0 + b) * mmax + m0 + a]);
+                G[(size_t)(m0 + a) * mmax + m0 + b] = G[(size_t)(m0 + b) * mmax + m0 + a] = x;
+            }
+        m = m0 + (int)keep.size();
+        return (int)keep.size();
+    }
+
     void update_G(int c0, int cnt) {
         std::vector<double> h = gram_host(V.p, c0 + cnt, W.p + (i64)c0 * nd, cnt);  // [(c0+cnt)][cnt]
         for (int i = 0; i < c0 + cnt; ++i)
@@ -405,20 +484,47 @@ struct Davidson {
             for (int j = 0; j < k; ++j)
                 if (!(res[j] <= tol * std::max(std::fabs(theta[j]), eps23))) add.push_back(j);
             if (m + (int)add.size() > mmax) {
-                // restart on the current Ritz vectors
-                PE_HIP(hipMemcpyAsync(V.p, X.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
-                PE_HIP(hipMemcpyAsync(W.p, WX.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
+                // thick restart on the lowest kk Ritz vectors (kk = 2k when room is left
+                // for the next corrections): they are orthonormal and G becomes diag(w)
+                const int kk = std::max(k, std::min({m, 2 * k, mmax - (int)add.size()}));
+                if (kk > k) {
+                    if (!RV.p) {
+                        RV.alloc((size_t)(nd * 2 * k));
+                        RW.alloc((size_t)(nd * 2 * k));
+                    }
+                    std::vector<double> ykk((size_t)m * kk);
+                    for (int j = 0; j < kk; ++j)
+                        for (int i = 0; i < m; ++i) ykk[(size_t)j * m + i] = y[(size_t)i * m + j];
+                    double *dYk = small.p;
+                    PE_HIP(hipMemcpyAsync(dYk, ykk.data(), sizeof(double) * ykk.size(), hipMemcpyHostToDevice, s));
+                    PE_HIP(hipMemsetAsync(RV.p, 0, sizeof(double) * nd * kk, s));
+                    PE_HIP(hipMemsetAsync(RW.p, 0, sizeof(double) * nd * kk, s));
+                    k_gemm_update<<<g1(), 256, 0, s>>>(nd, V.p, nd, m, dYk, kk, 1.0, RV.p, nd);
+                    PE_LAUNCH_CHECK();
+                    k_gemm_update<<<g1(), 256, 0, s>>>(nd, W.p, nd, m, dYk, kk, 1.0, RW.p, nd);
+                    PE_LAUNCH_CHECK();
+                    PE_HIP(hipMemcpyAsync(V.p, RV.p, sizeof(double) * nd * kk, hipMemcpyDeviceToDevice, s));
+                    PE_HIP(hipMemcpyAsync(W.p, RW.p, sizeof(double) * nd * kk, hipMemcpyDeviceToDevice, s));
+                    PE_HIP(hipStreamSynchronize(s));  // ykk (host) must outlive the copy
+                } else {
+                    PE_HIP(hipMemcpyAsync(V.p, X.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
+                    PE_HIP(hipMemcpyAsync(W.p, WX.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
+                }
                 std::fill(G.begin(), G.end(), 0.0);
-                for (int j = 0; j < k; ++j) G[(size_t)j * mmax + j] = theta[j];
-                m = k;
+                for (int j = 0; j < kk; ++j) G[(size_t)j * mmax + j] = w[(size_t)j];
+                m = kk;
             }
             const int m0 = m;
+            int slot = 0;
             for (int j : add) {
                 if (m >= mmax || m >= nd) break;
                 PE_HIP(hipMemcpyAsync(V.p + (i64)m * nd, T.p + (i64)j * nd, sizeof(double) * nd,
                                       hipMemcpyDeviceToDevice, s));
-                std::vector<double> n0 = gram_host(V.p + (i64)m * nd, 1, V.p + (i64)m * nd, 1);
-                if (orthonormalize_new(std::sqrt(std::max(n0[0], 0.0)))) ++m;
+                orthonormalize_device(slot++);
+            }
+            if (m > m0) {
+                apply(m0, m - m0, fp32);
+                update_G_checked(m0, m - m0);
             }
             if (m == m0) {
                 // no usable direction: accept if the residuals are tiny, else fail
@@ -427,8 +533,6 @@ struct Davidson {
                     if (!(res[j] <= 1e-7 * std::max(std::fabs(theta[j]), 1.0))) tiny = false;
                 return tiny && !fp32;
             }
-            apply(m0, m - m0, fp32);
-            update_G(m0, m - m0);
         }
     }
 };
@@ -439,7 +543,10 @@ void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, 
     hipStream_t s = op->ctx->stream;
     const int k = (int)n;
     PE_REQUIRE(k <= 16, PYCI_EUNSUPPORTED, "solve supports at most 16 roots per call");
-    i64 mm = (ncv > 0) ? ncv : std::min<i64>(nd, std::max<i64>(2 * n + 1, 20));
+    // default subspace max(10, 2n+1, 6n): thick restarts keep 2n Ritz vectors, so a
+    // single root converges as fast as with the reference's Lanczos default of 20
+    // while each iteration streams half the basis; blocks get 6n (fewer restarts)
+    i64 mm = (ncv > 0) ? ncv : std::min<i64>(nd, std::max<i64>({2 * n + 1, 10, 6 * n}));
     mm = std::max<i64>(mm, 2 * n + 1);
     mm = std::min<i64>(mm, nd);
     mm = std::max<i64>(mm, n);
