@@ -51,7 +51,119 @@ WORKLOADS = {
     "fci8": dict(name="fci8", nbasis=8, nup=4, ndn=4, integrals="fci8", nroot=1),
     # nbasis 14: the CSR (39.8 G nonzeros) does not fit in HBM; matrix-free only
     "fci14": dict(name="fci14", nbasis=14, nup=7, ndn=7, integrals="fci14", nroot=8, csr=False),
+    # 64 uneven FullCI problems (nbasis 8..12, 3..6 electrons of each spin), 2 roots each
+    "batch64": dict(name="batch64", nroot=2, batch=64),
 }
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("graph_launches", ctypes.c_int64), ("rows_per_block", ctypes.c_int64), ("converged", ctypes.c_int64),
+                ("setup_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
+def bench_batch(args, wl):
+    """Batched uneven solves: throughput and per-problem latency, every root
+    checked against a per-problem FP64 solve of the same operator."""
+    from pyci_amd.synthetic import batch_problems
+
+    L = engine()
+    ctx = ctypes.c_void_p()
+    ok(L, L.pyci_ctx_create(0, ctypes.byref(ctx)), "pyci_ctx_create")
+    probs = batch_problems(count=wl["batch"])
+    npb = len(probs)
+    nroot = args.nroot or wl["nroot"]
+    i64a = lambda v: np.ascontiguousarray(v, dtype=np.int64)
+    nb = i64a([q["nbasis"] for q in probs])
+    nu = i64a([q["nocc"] for q in probs])
+    ec = np.zeros(npb)
+    ones = [np.ascontiguousarray(q["one"]) for q in probs]
+    twos = [np.ascontiguousarray(q["two"]) for q in probs]
+    PP = ctypes.POINTER(ctypes.c_double) * npb
+    op = PP(*[dp(a) for a in ones])
+    tp = PP(*[dp(a) for a in twos])
+    L.pyci_batch_solve.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_double,
+                                   ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(BatchStats)]
+    prec = 0 if args.precision == "f64" else 1
+
+    def run():
+        ev = np.zeros((npb, nroot))
+        lat = np.zeros(npb)
+        it = np.zeros(npb, dtype=np.int64)
+        ns = np.zeros(npb, dtype=np.int64)
+        cv = np.zeros(npb, dtype=np.int32)
+        st = BatchStats()
+        ok(L, L.pyci_batch_solve(ctx, npb, nb.ctypes.data, nu.ctypes.data, nu.ctypes.data, ec.ctypes.data, op, tp,
+                                 nroot, args.tol, prec, -1, ev.ctypes.data, lat.ctypes.data, it.ctypes.data,
+                                 ns.ctypes.data, cv.ctypes.data, ctypes.byref(st)), "batch_solve")
+        return ev, lat, it, ns, cv, st
+
+    # per-problem FP64 references (untimed): the matrix-free operator, tol 1e-10
+    from math import comb
+
+    e_ref = np.zeros((npb, nroot))
+    L.pyci_op_create_ex.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                                    ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    t0 = time.perf_counter()
+    for p_, q in enumerate(probs):
+        n_, m_ = q["nbasis"], q["nocc"]
+        ndet = comb(n_, m_) ** 2
+        dets = np.zeros((ndet, 2), dtype=np.uint64)
+        ok(L, L.pyci_fill_all_dets(ctx, 1, n_, m_, m_, dets.ctypes.data_as(ctypes.c_void_p)), "fill")
+        ham, wfn, o = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        ok(L, L.pyci_ham_create(ctx, n_, 0.0, dp(ones[p_]), dp(twos[p_]), ctypes.byref(ham)), "ham")
+        ok(L, L.pyci_wfn_create(ctx, 1, n_, m_, m_, ndet, dets.ctypes.data_as(ctypes.c_void_p), ctypes.byref(wfn)),
+           "wfn")
+        ok(L, L.pyci_op_create_ex(ham, wfn, -1, -1, 1, 2, ctypes.byref(o)), "op")
+        ev = np.zeros(nroot)
+        vec = np.zeros((nroot, ndet))
+        st = Stats()
+        ok(L, L.pyci_op_solve(o, nroot, None, -1, -1, 1e-10, 0, dp(ev), dp(vec), ctypes.byref(st)), "solve")
+        e_ref[p_] = ev
+        L.pyci_op_destroy(o)
+        L.pyci_wfn_destroy(wfn)
+        L.pyci_ham_destroy(ham)
+    log(f"per-problem FP64 references: {time.perf_counter() - t0:.1f} s")
+    for w in range(args.warmup):
+        run()
+    res = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        ev, lat, it, ns, cv, st = run()
+        t = time.perf_counter() - t0
+        de = float(np.max(np.abs(ev - e_ref)))
+        res.append((t, ev, lat, it, ns, cv, st, de))
+        log(f"step {k}: {t*1e3:.1f} ms ({st.setup_ms:.1f} ms set-up), {int(cv.sum())}/{npb} converged, "
+            f"max |dE| {de:.2e}, {st.graph_launches} iterations, rows/block {st.rows_per_block}")
+    tt = float(np.mean([r[0] for r in res]))
+    lat = np.concatenate([r[2] for r in res])
+    dims = [comb(q["nbasis"], q["nocc"]) ** 2 for q in probs]
+    out = {
+        "metric": "batched uneven FullCI solves: problems/s; per-problem latency p50/p99",
+        "value": round(npb / tt, 3),
+        "unit": "problems/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == 0 else "f32-sigma+f64",
+        "data": "synthetic (seeded integral scan)",
+        "config": {"workload": f"{npb} FullCI problems, nbasis 8..12, 3..6 electrons of each spin, "
+                               f"dimensions {min(dims)}..{max(dims)} (sum {sum(dims)}), nroot={nroot}",
+                   "precision": args.precision, "tol": args.tol, "parallelism": "single GPU"},
+        "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+        "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+        "setup_ms": round(float(np.mean([r[6].setup_ms for r in res])), 3),
+        "iterations_max": int(max(r[6].graph_launches for r in res)),
+        "rows_per_block": int(res[-1][6].rows_per_block),
+        "converged": int(min(r[5].sum() for r in res)),
+        "max_abs_dE": max(r[7] for r in res),
+    }
+    print(json.dumps(out), flush=True)
 
 
 def log(*a):
@@ -78,6 +190,8 @@ def engine():
     L.pyci_op_bench_sigma_parts.argtypes = [vp, i64, i64, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_float), dp, dp]
     L.pyci_op_destroy.argtypes = [vp]
+    L.pyci_wfn_destroy.argtypes = [vp]
+    L.pyci_ham_destroy.argtypes = [vp]
     L.pyci_op_info.argtypes = [vp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                ctypes.c_void_p, ctypes.c_void_p]
     L.pyci_op_solve.argtypes = [vp, i64, dp, i64, i64, ctypes.c_double, ctypes.c_int, dp, dp, ctypes.POINTER(Stats)]
@@ -190,6 +304,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     wl = WORKLOADS[args.workload]
+    if "batch" in wl:
+        bench_batch(args, wl)
+        return
     L = engine()
     ctx = ctypes.c_void_p()
     ok(L, L.pyci_ctx_create(0, ctypes.byref(ctx)), "pyci_ctx_create")

@@ -217,6 +217,34 @@ int pyci_mlp_train(pyci_mlp *m, const pyci_wfn *wfn, const double *coeffs, doubl
 int pyci_add_hci_mlp(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs, double eps, pyci_mlp *m, double pool,
                      int64_t nadd, int score_mode, int64_t *nadded);
 
+/* ------------------------------------------------------------------------
+ * Batched solves of many uneven FullCI problems. No counterpart in the
+ * reference: each problem is SparseOp(ham, wfn) + solve_ci(n) (sparseop.cpp:
+ * 114-146) on the full FullCI space of its own integrals (add_all_dets), and all
+ * problems run together on the device (matrix-free sigma, one kernel launch per
+ * Davidson step over all problems, the iteration replayed as a hipGraph).
+ * ------------------------------------------------------------------------ */
+typedef struct pyci_batch_stats {
+    int64_t graph_launches; /* Davidson iterations replayed (the batch's maximum) */
+    int64_t rows_per_block; /* autotuned rows per workgroup of the tall-skinny kernels */
+    int64_t converged;      /* problems that converged */
+    double setup_ms;        /* host set-up (tables, integrals, diagonals) to the first iteration */
+    double total_ms;        /* whole call */
+} pyci_batch_stats;
+
+/* nprob problems: nbasis[p] <= 15 spatial orbitals, nocc_up[p] >= nocc_dn[p] >= 1,
+ * ecore[p], one_mo[p] (n,n) and two_mo[p] (n,n,n,n) host arrays as for
+ * pyci_ham_create. nroot <= 8 lowest roots each, tol as pyci_op_solve, precision
+ * PYCI_PREC_F64 or PYCI_PREC_MIXED. Outputs (host, may be NULL except evals):
+ * evals[p * nroot + j] (ecore added; descending for nroot > 1 as solve_ci),
+ * latency_ms[p] from the first iteration to the problem's convergence (device
+ * clock), iterations[p], nsigma[p] (vectors multiplied by H), converged[p]. */
+int pyci_batch_solve(pyci_ctx *ctx, int64_t nprob, const int64_t *nbasis, const int64_t *nocc_up,
+                     const int64_t *nocc_dn, const double *ecore, const double *const *one_mo,
+                     const double *const *two_mo, int64_t nroot, double tol, int precision, int64_t maxiter,
+                     double *evals, double *latency_ms, int64_t *iterations, int64_t *nsigma, int *converged,
+                     pyci_batch_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

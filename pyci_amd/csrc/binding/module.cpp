@@ -431,6 +431,57 @@ PYBIND11_MODULE(_pyci, m) {
     m.def("compute_rdms", [](DOCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));
     m.def("compute_rdms", [](FullCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));
     m.def("compute_rdms", [](GenCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));
+    m.def(
+        "batch_solve",
+        [](py::list problems, i64 n, double tol, const std::string &precision, i64 maxiter) {
+            const i64 np = (i64)py::len(problems);
+            if (np == 0) throw py::value_error("no problems");
+            int prec;
+            if (precision == "f64") prec = PYCI_PREC_F64;
+            else if (precision == "mixed") prec = PYCI_PREC_MIXED;
+            else throw py::value_error("precision must be 'f64' or 'mixed'");
+            std::vector<i64> nb(np), nu(np), nd(np);
+            std::vector<double> ec(np);
+            std::vector<Array<double>> ones, twos;
+            std::vector<const double *> op(np), tp(np);
+            for (i64 p = 0; p < np; ++p) {
+                py::tuple t = problems[p].cast<py::tuple>();
+                if (t.size() != 3) throw py::value_error("each problem is (hamiltonian, nocc_up, nocc_dn)");
+                SQuantOp &h = t[0].cast<SQuantOp &>();
+                nb[p] = h.nbasis;
+                nu[p] = t[1].cast<i64>();
+                nd[p] = t[2].cast<i64>();
+                ec[p] = h.ecore;
+                ones.push_back(h.one_mo);
+                twos.push_back(h.two_mo);
+                op[p] = ones.back().data();
+                tp[p] = twos.back().data();
+            }
+            Array<double> ev({np, n}), lat(np);
+            Array<i64> it(np), ns(np);
+            std::vector<int> conv(np);
+            pyci_batch_stats st{};
+            {
+                py::gil_scoped_release nogil;
+                check(pyci_batch_solve(context(), np, nb.data(), nu.data(), nd.data(), ec.data(), op.data(),
+                                       tp.data(), n, tol, prec, maxiter, ev.mutable_data(), lat.mutable_data(),
+                                       it.mutable_data(), ns.mutable_data(), conv.data(), &st));
+            }
+            py::dict info;
+            info["latency_ms"] = lat;
+            info["iterations"] = it;
+            info["sigma_vectors"] = ns;
+            py::list cv;
+            for (int c : conv) cv.append(c != 0);
+            info["converged"] = cv;
+            info["graph_launches"] = st.graph_launches;
+            info["rows_per_block"] = st.rows_per_block;
+            info["setup_ms"] = st.setup_ms;
+            info["total_ms"] = st.total_ms;
+            return py::make_tuple(ev, info);
+        },
+        py::arg("problems"), py::arg("n") = 2, py::arg("tol") = 1.0e-6, py::arg("precision") = "mixed",
+        py::arg("maxiter") = -1);
     py::class_<MLPSelector>(m, "mlp_selector")
         .def(py::init<i64, u64>(), py::arg("nfeat"), py::arg("seed") = 0)
         .def_readonly("nfeat", &MLPSelector::nfeat)

@@ -40,6 +40,13 @@ static int guarded(F &&f) {
 
 static void set_device(pyci_ctx *ctx) { PE_HIP(hipSetDevice(ctx->device)); }
 
+// swap_nb of an operator over wfn's rows (see pyci_op::swap_nb)
+static i64 swap_partner_nb(const pyci_op *op, const pyci_wfn *wfn) {
+    const bool full = wfn->kind == PYCI_WFN_FULLCI && wfn->colex && wfn->nup == wfn->ndn &&
+                      op->nrow == wfn->ndet && op->ncol == wfn->ndet && op->symmetric;
+    return full ? wfn->nsd : 0;
+}
+
 static void validate_kind(int kind, i64 nb, i64 nup, i64 ndn) {
     PE_REQUIRE(kind == PYCI_WFN_DOCI || kind == PYCI_WFN_FULLCI || kind == PYCI_WFN_GENCI, PYCI_EINVAL,
                "unknown wave function kind");
@@ -297,6 +304,7 @@ int pyci_op_create_ex(pyci_ham *ham, pyci_wfn *wfn, int64_t nrow, int64_t ncol, 
             } else {
                 op_build_all(op, ham, wfn);
             }
+            op->swap_nb = swap_partner_nb(op, wfn);
         } catch (...) {
             delete op;
             throw;
@@ -321,8 +329,10 @@ int pyci_op_update(pyci_op *op, pyci_ham *ham, pyci_wfn *wfn) {
             op->nrow = wfn->ndet;
             op->ncol = wfn->ndet;
             op_build_all(op, ham, wfn);
+            op->swap_nb = swap_partner_nb(op, wfn);
             return;
         }
+        op->swap_nb = 0;
         const i64 old = op->nrow;
         if (op->symmetric) {
             // The lower-triangle export of rows [0, old) is unchanged by new columns
@@ -331,6 +341,7 @@ int pyci_op_update(pyci_op *op, pyci_ham *ham, pyci_wfn *wfn) {
             op->nrow = wfn->ndet;
             op->ncol = wfn->ndet;
             op_build_all(op, ham, wfn);
+            op->swap_nb = swap_partner_nb(op, wfn);
             return;
         }
         // Non-symmetric: old rows keep their column bound, new rows use ndet
@@ -528,6 +539,22 @@ int pyci_compute_rdms(pyci_wfn *wfn, const double *coeffs, double *rdm1, double 
     return guarded([&] {
         set_device(wfn->ctx);
         rdms_device(wfn, coeffs, rdm1, rdm2);
+    });
+}
+
+int pyci_batch_solve(pyci_ctx *ctx, int64_t nprob, const int64_t *nbasis, const int64_t *nocc_up,
+                     const int64_t *nocc_dn, const double *ecore, const double *const *one_mo,
+                     const double *const *two_mo, int64_t nroot, double tol, int precision, int64_t maxiter,
+                     double *evals, double *latency_ms, int64_t *iterations, int64_t *nsigma, int *converged,
+                     pyci_batch_stats *stats) {
+    return guarded([&] {
+        PE_REQUIRE(ctx && nbasis && nocc_up && nocc_dn && ecore && one_mo && two_mo && evals, PYCI_EINVAL,
+                   "null argument");
+        PE_REQUIRE(precision == PYCI_PREC_F64 || precision == PYCI_PREC_MIXED, PYCI_EINVAL, "unknown precision");
+        set_device(ctx);
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        batch_solve(ctx, nprob, nbasis, nocc_up, nocc_dn, ecore, one_mo, two_mo, nroot, tol, precision, maxiter,
+                    evals, latency_ms, iterations, nsigma, converged, stats);
     });
 }
 

@@ -157,6 +157,15 @@ __global__ void k_scale_checked(double *x, i64 n, const double *nn) {
     for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) x[i] *= s;
 }
 
+// dst = (src + s P src) / 2 with P the alpha <-> beta exchange of a full FullCI
+// space in colex order with nup == ndn: index Ja * nb + Jb <-> Jb * nb + Ja.
+__global__ void k_parity(const double *src, double *dst, i64 nd, i64 nb, double sgn) {
+    for (i64 J = (i64)blockIdx.x * blockDim.x + threadIdx.x; J < nd; J += (i64)gridDim.x * blockDim.x) {
+        const i64 a = J / nb, b = J - a * nb;
+        dst[J] = 0.5 * (src[J] + sgn * src[b * nb + a]);
+    }
+}
+
 // col-major (nd x k, ld) <-> row-major (nd x k), with optional precision change
 template <typename S, typename D>
 __global__ void k_col2row(i64 nd, int k, const S *src, i64 ld, D *dst) {
@@ -235,6 +244,8 @@ struct Davidson {
     std::vector<double> G;  // mmax x mmax row-major
     int m = 0;
     int nblk;
+    int sector = 0;   // 0, or +1 / -1: the basis is kept in that parity sector of P
+    i64 swap_nb = 0;  // beta-string count of P's index map
     pyci_solve_stats st{};
 
     Davidson(pyci_op *o, int k_, int mmax_) : op(o), s(o->ctx->stream), nd(o->nrow), k(k_), mmax(mmax_) {
@@ -243,8 +254,10 @@ struct Davidson {
         X.alloc((size_t)(nd * k));
         WX.alloc((size_t)(nd * k));
         T.alloc((size_t)(nd * k));
-        rowbuf.alloc((size_t)(nd * k));
-        rowout.alloc((size_t)(nd * k));
+        // staging for sigma of up to max(k, start block) vectors at once
+        const int kcols = std::max(k, std::min(2 * k, 16));
+        rowbuf.alloc((size_t)(nd * kcols));
+        rowout.alloc((size_t)(nd * kcols));
         nblk = (int)std::min<i64>(512, (nd + DT - 1) / DT);
         partial.alloc((size_t)nblk * (size_t)std::max(mmax * k, 64) + 64);
         small.alloc((size_t)(mmax * mmax + 4 * mmax + 64));
@@ -298,8 +311,8 @@ struct Davidson {
         double *dst = W.p + (i64)c0 * nd;
         if (fp32) {
             if (!row32.p || row32.n < (size_t)(nd * cnt)) {
-                row32.alloc((size_t)(nd * k));
-                out32.alloc((size_t)(nd * k));
+                row32.alloc((size_t)(nd * std::max(cnt, k)));
+                out32.alloc((size_t)(nd * std::max(cnt, k)));
             }
             k_col2row<double, float><<<g1(), 256, 0, s>>>(nd, cnt, src, nd, row32.p);
             PE_LAUNCH_CHECK();
@@ -470,13 +483,17 @@ struct Davidson {
                     improved = true;
                 }
             stall = improved ? 0 : stall + 1;
-            if (stall >= (fp32 ? 8 : 25)) {
+            // stagnation: the FP32 phase hands over after 8 flat iterations; in FP64 a
+            // residual at the rounding floor is accepted after 25, and only 300 flat
+            // iterations count as failure (slow roots can plateau for dozens of
+            // iterations before they converge)
+            if (fp32 && stall >= 8) return false;
+            if (!fp32 && stall >= 25) {
                 bool tiny = true;
                 for (int j = 0; j < k; ++j)
                     if (!(res[j] <= 1e-7 * std::max(std::fabs(theta[j]), 1.0))) tiny = false;
-                if (fp32) return false;
                 if (tiny) return true;
-                return false;
+                if (stall >= 300) return false;
             }
             if (iters >= maxiter) return false;
             // unconverged roots contribute corrections
@@ -518,8 +535,14 @@ struct Davidson {
             int slot = 0;
             for (int j : add) {
                 if (m >= mmax || m >= nd) break;
-                PE_HIP(hipMemcpyAsync(V.p + (i64)m * nd, T.p + (i64)j * nd, sizeof(double) * nd,
-                                      hipMemcpyDeviceToDevice, s));
+                if (sector != 0) {
+                    // keep the basis in its sector (rounding would otherwise let it drift)
+                    k_parity<<<g1(), 256, 0, s>>>(T.p + (i64)j * nd, V.p + (i64)m * nd, nd, swap_nb, (double)sector);
+                    PE_LAUNCH_CHECK();
+                } else {
+                    PE_HIP(hipMemcpyAsync(V.p + (i64)m * nd, T.p + (i64)j * nd, sizeof(double) * nd,
+                                          hipMemcpyDeviceToDevice, s));
+                }
                 orthonormalize_device(slot++);
             }
             if (m > m0) {
@@ -537,59 +560,93 @@ struct Davidson {
     }
 };
 
-void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, double tol, int precision,
-                    double *evals, double *evecs, pyci_solve_stats *stats) {
+// One Davidson run for the k lowest roots (of a parity sector of P when sector
+// != 0). Returns converged flag; theta (k) ascending and the Ritz vectors (k x nd,
+// host) of the run.
+static bool solve_run(pyci_op *op, int k, const double *c0, i64 mm, i64 maxit, double tol, int precision, int sector,
+                      std::vector<double> &theta, std::vector<double> &hx, pyci_solve_stats &st) {
     const i64 nd = op->nrow;
     hipStream_t s = op->ctx->stream;
-    const int k = (int)n;
-    PE_REQUIRE(k <= 16, PYCI_EUNSUPPORTED, "solve supports at most 16 roots per call");
-    // default subspace max(10, 2n+1, 6n): thick restarts keep 2n Ritz vectors, so a
-    // single root converges as fast as with the reference's Lanczos default of 20
-    // while each iteration streams half the basis; blocks get 6n (fewer restarts)
-    i64 mm = (ncv > 0) ? ncv : std::min<i64>(nd, std::max<i64>({2 * n + 1, 10, 6 * n}));
-    mm = std::max<i64>(mm, 2 * n + 1);
-    mm = std::min<i64>(mm, nd);
-    mm = std::max<i64>(mm, n);
-    const i64 maxit = (maxiter > 0) ? maxiter : std::max<i64>(1000, std::min<i64>(n * nd * 10, 100000));
-    if (precision == PYCI_PREC_MIXED) ensure_vals32(op);
+    const i64 nb = op->swap_nb;
+    auto partner = [&](i64 J) { return (J % nb) * nb + J / nb; };
     Davidson D(op, k, (int)mm);
-    // initial block: c0 (if given) then unit vectors on the lowest diagonal
-    // entries, each mixed with a small fixed pseudo-random vector so that no
-    // symmetry sector is excluded.
+    D.sector = sector;
+    D.swap_nb = nb;
     std::vector<double> hd((size_t)nd);
     PE_HIP(hipMemcpyAsync(hd.data(), op->diag.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
+    // start block: unit vectors on the lowest diagonal entries (in a sector: the
+    // sector's combination of a determinant and its partner), each with a small
+    // fixed perturbation; for several roots twice as many vectors as roots (at most
+    // 16, leaving room for k corrections)
+    const int kb0 = (k == 1) ? 1 : (int)std::min<i64>(nd, std::max<i64>(k, std::min<i64>({(i64)2 * k, 16, mm - k})));
+    const i64 want = std::min<i64>(nd, 4 * (i64)kb0 + 64);
     std::vector<i64> idx((size_t)nd);
     std::iota(idx.begin(), idx.end(), 0);
-    const i64 want = std::min<i64>(nd, k + 1);
-    std::partial_sort(idx.begin(), idx.begin() + want, idx.end(),
-                      [&](i64 a, i64 b) { return hd[(size_t)a] < hd[(size_t)b] || (hd[(size_t)a] == hd[(size_t)b] && a < b); });
-    std::vector<double> init((size_t)nd);
-    int used = 0;
+    std::partial_sort(idx.begin(), idx.begin() + want, idx.end(), [&](i64 a, i64 b) {
+        return hd[(size_t)a] < hd[(size_t)b] || (hd[(size_t)a] == hd[(size_t)b] && a < b);
+    });
+    std::vector<double> init((size_t)nd), noise((size_t)nd);
+    std::vector<char> taken;
+    if (sector != 0) taken.assign((size_t)nd, 0);
+    size_t cursor = 0;
     D.m = 0;
-    for (int j = 0; j < k; ++j) {
-        if (j == 0 && c0) {
+    // the caller's start vector (its part in this sector), unless that part vanishes
+    bool use_c0 = c0 != nullptr;
+    if (use_c0 && sector != 0) {
+        double all = 0.0, part = 0.0;
+        for (i64 r = 0; r < nd; ++r) {
+            const double x = 0.5 * (c0[r] + sector * c0[partner(r)]);
+            all += c0[r] * c0[r];
+            part += x * x;
+        }
+        use_c0 = part > 1e-16 * all && part > 0.0;
+    }
+    for (int j = 0; D.m < kb0 && j < kb0 + 64; ++j) {
+        bool have = false;
+        if (j == 0 && use_c0) {
             std::memcpy(init.data(), c0, sizeof(double) * nd);
+            if (sector != 0)
+                for (i64 r = 0; r < nd; ++r) init[(size_t)r] = 0.5 * (c0[r] + sector * c0[partner(r)]);
+            have = true;
         } else {
-            // perturbation of total norm ~1e-3: seeds every symmetry sector without
-            // moving the start away from the dominant determinant (a perturbation of
-            // norm ~0.6 leaves a start whose Rayleigh quotient is far above the
-            // lowest diagonal element and can converge to an excited root)
+            // perturbation of total norm ~1e-3: seeds every direction without moving
+            // the start away from the dominant determinant (a perturbation of norm
+            // ~0.6 leaves a start whose Rayleigh quotient is far above the lowest
+            // diagonal element and can converge to an excited root)
             u64 seed = 0x243f6a8885a308d3ULL + (u64)j;
             const double amp = 2e-3 * std::sqrt(3.0 / (double)nd);
-            for (i64 r = 0; r < nd; ++r) init[(size_t)r] = amp * ((double)(mix64(seed + (u64)r) >> 11) * 0x1p-53 - 0.5);
-            init[(size_t)idx[(size_t)std::min<i64>(used, nd - 1)]] += 1.0;
-            ++used;
+            for (i64 r = 0; r < nd; ++r) noise[(size_t)r] = amp * ((double)(mix64(seed + (u64)r) >> 11) * 0x1p-53 - 0.5);
+            for (; cursor < (size_t)want && !have; ++cursor) {
+                const i64 d0 = idx[cursor];
+                if (sector == 0) {
+                    init = noise;
+                    init[(size_t)d0] += 1.0;
+                    have = true;
+                    continue;
+                }
+                const i64 d1 = partner(d0);
+                if (taken[(size_t)d0] || (d1 == d0 && sector < 0)) continue;
+                taken[(size_t)d0] = taken[(size_t)d1] = 1;
+                for (i64 r = 0; r < nd; ++r) init[(size_t)r] = 0.5 * (noise[(size_t)r] + sector * noise[(size_t)partner(r)]);
+                if (d1 == d0) {
+                    init[(size_t)d0] += 1.0;
+                } else {
+                    init[(size_t)d0] += std::sqrt(0.5);
+                    init[(size_t)d1] += sector * std::sqrt(0.5);
+                }
+                have = true;
+            }
         }
+        if (!have) continue;
         PE_HIP(hipMemcpyAsync(D.V.p + (i64)D.m * nd, init.data(), sizeof(double) * nd, hipMemcpyHostToDevice, s));
         std::vector<double> n0 = D.gram_host(D.V.p + (i64)D.m * nd, 1, D.V.p + (i64)D.m * nd, 1);
-        if (D.orthonormalize_new(std::sqrt(std::max(n0[0], 0.0)))) ++D.m;
+        if (std::sqrt(std::max(n0[0], 0.0)) > 1e-12 && D.orthonormalize_new(std::sqrt(std::max(n0[0], 0.0)))) ++D.m;
     }
-    PE_REQUIRE(D.m == k, PYCI_ENOCONV, "did not converge");
+    PE_REQUIRE(D.m >= k, PYCI_ENOCONV, "did not converge");
     const bool mixed = (precision == PYCI_PREC_MIXED);
     D.apply(0, D.m, mixed);
     D.update_G(0, D.m);
-    std::vector<double> theta;
     i64 iters = 0;
     bool ok;
     if (mixed) {
@@ -612,16 +669,70 @@ void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, 
     } else {
         ok = D.run(tol, maxit, false, 0.0, theta, iters);
     }
-    if (stats) *stats = D.st;
-    PE_REQUIRE(ok, PYCI_ENOCONV, "did not converge");
-    // reference order: descending for n > 1 (sparseop.cpp:134-144)
-    std::vector<double> hx((size_t)(nd * k));
+    st = D.st;
+    hx.resize((size_t)(nd * k));
     PE_HIP(hipMemcpyAsync(hx.data(), D.X.p, sizeof(double) * nd * k, hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
+    return ok;
+}
+
+void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, double tol, int precision,
+                    double *evals, double *evecs, pyci_solve_stats *stats) {
+    const i64 nd = op->nrow;
+    const int k = (int)n;
+    PE_REQUIRE(k <= 16, PYCI_EUNSUPPORTED, "solve supports at most 16 roots per call");
+    // default subspace max(10, 2n+1, 6n): thick restarts keep 2n Ritz vectors, so a
+    // single root converges as fast as with the reference's Lanczos default of 20
+    // while each iteration streams half the basis; blocks get 6n (fewer restarts)
+    i64 mm = (ncv > 0) ? ncv : std::min<i64>(nd, std::max<i64>({2 * n + 1, 10, 6 * n}));
+    mm = std::max<i64>(mm, 2 * n + 1);
+    mm = std::min<i64>(mm, nd);
+    mm = std::max<i64>(mm, n);
+    const i64 maxit = (maxiter > 0) ? maxiter : std::max<i64>(1000, std::min<i64>(n * nd * 10, 100000));
+    if (precision == PYCI_PREC_MIXED) ensure_vals32(op);
+    // The alpha <-> beta exchange P commutes with H and with the diagonal
+    // preconditioner, so a Davidson basis never leaves the parity sectors its start
+    // vectors occupy, and roots of a sector that is not among the k lowest Ritz pairs
+    // are never targeted (a random-start Lanczos, as the reference's, has no such
+    // blind spot). Where P is known (full FullCI space, nup == ndn) both sectors are
+    // solved and the k lowest roots of the two are returned.
+    const i64 nb = op->swap_nb;
+    const i64 odd_dim = (nb > 0) ? (nd - nb) / 2 : 0;
+    const bool sectors = nb > 0 && odd_dim >= 2 * mm && nd - odd_dim >= 2 * mm;
+    std::vector<double> theta_a, hx_a, theta_b, hx_b;
+    pyci_solve_stats st_a{}, st_b{};
+    std::vector<std::pair<double, std::pair<int, int>>> roots;  // (theta, (run, index))
+    bool ok_a, ok_b = true;
+    if (sectors) {
+        ok_a = solve_run(op, k, c0, mm, maxit, tol, precision, +1, theta_a, hx_a, st_a);
+        ok_b = solve_run(op, k, c0, mm, maxit, tol, precision, -1, theta_b, hx_b, st_b);
+        for (int j = 0; j < k; ++j) {
+            roots.push_back({theta_a[(size_t)j], {0, j}});
+            roots.push_back({theta_b[(size_t)j], {1, j}});
+        }
+        std::stable_sort(roots.begin(), roots.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+        roots.resize((size_t)k);
+    } else {
+        ok_a = solve_run(op, k, c0, mm, maxit, tol, precision, 0, theta_a, hx_a, st_a);
+        for (int j = 0; j < k; ++j) roots.push_back({theta_a[(size_t)j], {0, j}});
+    }
+    if (stats) {
+        *stats = st_a;
+        stats->iterations += st_b.iterations;
+        stats->sigma_calls += st_b.sigma_calls;
+        stats->sigma_calls_f32 += st_b.sigma_calls_f32;
+        for (int j = 0; j < k && j < 64; ++j) {
+            const auto &r = roots[(size_t)j].second;
+            stats->residual[j] = (r.first == 0 ? st_a : st_b).residual[r.second];
+        }
+    }
+    PE_REQUIRE(ok_a && ok_b, PYCI_ENOCONV, "did not converge");
+    // reference order: descending for n > 1 (sparseop.cpp:134-144)
     for (int j = 0; j < k; ++j) {
-        const int src = (k > 1) ? (k - 1 - j) : 0;
-        evals[j] = theta[src] + op->ecore;
-        std::memcpy(evecs + (i64)j * nd, hx.data() + (i64)src * nd, sizeof(double) * nd);
+        const auto &r = roots[(size_t)((k > 1) ? (k - 1 - j) : 0)];
+        const std::vector<double> &hx = (r.second.first == 0) ? hx_a : hx_b;
+        evals[j] = r.first + op->ecore;
+        std::memcpy(evecs + (i64)j * nd, hx.data() + (i64)r.second.second * nd, sizeof(double) * nd);
     }
 }
 

@@ -95,10 +95,13 @@ __global__ void __launch_bounds__(256) k_mf_gemm(i64 ncol, const T *__restrict__
         acc[m][0] = acc_t{};
         acc[m][1] = acc_t{};
     }
+    // columns at or beyond ncol are read as zero: they stay zero in every row, so
+    // the zero pad rows never meet a non-finite value
+    const bool live0 = c0 + lr < ncol, live1 = c0 + 16 + lr < ncol;
 #pragma unroll 2
     for (int k0 = 0; k0 < KP; k0 += 4) {
-        const T b0 = W[(i64)(k0 + lk) * ldw + c0 + lr];
-        const T b1 = W[(i64)(k0 + lk) * ldw + c0 + 16 + lr];
+        const T b0 = live0 ? W[(i64)(k0 + lk) * ldw + c0 + lr] : (T)0;
+        const T b1 = live1 ? W[(i64)(k0 + lk) * ldw + c0 + 16 + lr] : (T)0;
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
             const T a = V[(16 * m + lr) * KP + k0 + lk];
@@ -205,10 +208,24 @@ __global__ void k_mf_diag(i64 nd, i64 NB, const u64 *__restrict__ sa, const u64 
     diag[J] = v2;
 }
 
+__global__ void k_wallclock(unsigned long long *t) {
+    if (threadIdx.x == 0) t[0] = wall_clock64();
+}
+
+void mf_diag_launch(i64 nd, i64 NB, const u64 *sa, const u64 *sb, HamView H, double *diag, hipStream_t s) {
+    k_mf_diag<<<grid_for(nd, 256), 256, 0, s>>>(nd, NB, sa, sb, H, diag);
+    PE_LAUNCH_CHECK();
+}
+
+void mf_wallclock_launch(unsigned long long *t, hipStream_t s) {
+    k_wallclock<<<1, 64, 0, s>>>(t);
+    PE_LAUNCH_CHECK();
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-static std::vector<u64> colex_strings(int n, int k) {
+std::vector<u64> colex_strings(int n, int k) {
     std::vector<u64> s;
     if (k == 0) {
         s.push_back(0);
@@ -225,7 +242,7 @@ static std::vector<u64> colex_strings(int n, int k) {
     return s;
 }
 
-static i64 rank_host(u64 det, const std::vector<i64> &binom) {
+i64 rank_host(u64 det, const std::vector<i64> &binom) {
     i64 r = 0;
     int k = 0;
     while (det) {
@@ -237,7 +254,7 @@ static i64 rank_host(u64 det, const std::vector<i64> &binom) {
     return r;
 }
 
-static std::vector<int2> string_table(int n, const std::vector<u64> &S, int ne, const std::vector<i64> &binom) {
+std::vector<int2> string_table(int n, const std::vector<u64> &S, int ne, const std::vector<i64> &binom) {
     std::vector<int2> t(S.size() * (size_t)ne);
     std::vector<std::pair<int, int2>> row;
     for (size_t r = 0; r < S.size(); ++r) {
@@ -264,6 +281,23 @@ static std::vector<int2> string_table(int n, const std::vector<u64> &S, int ne, 
         for (int e = 0; e < ne; ++e) t[r * ne + e] = row[(size_t)e].second;
     }
     return t;
+}
+
+// (ij|kl) over packed pairs (zero padded to kp x kp) and k_ij = h_ij - 1/2 sum_r (ir|rj).
+void pair_integrals(int n, const double *one, const double *two, int kp, std::vector<double> &V,
+                    std::vector<double> &kv) {
+    V.assign((size_t)kp * kp, 0.0);
+    kv.assign((size_t)kp, 0.0);
+    auto g = [&](int p, int q, int r, int t) { return two[(((size_t)p * n + q) * n + r) * n + t]; };
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            const int p = i * (i + 1) / 2 + j;
+            double kk = one[(size_t)i * n + j];
+            for (int r = 0; r < n; ++r) kk -= 0.5 * g(i, r, r, j);  // (ir|rj) = <ir|rj>
+            kv[(size_t)p] = kk;
+            for (int k = 0; k < n; ++k)
+                for (int l = 0; l <= k; ++l) V[(size_t)p * kp + k * (k + 1) / 2 + l] = g(i, k, j, l);  // (ij|kl)
+        }
 }
 
 bool direct_applicable(const pyci_ham *ham, const pyci_wfn *wfn, i64 nrow, i64 ncol, int symmetric) {
@@ -317,17 +351,8 @@ void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn) {
     PE_HIP(hipMemcpyAsync(D->one_mo.p, one.data(), sizeof(double) * n2, hipMemcpyHostToDevice, s));
     PE_HIP(hipMemcpyAsync(D->two_mo.p, two.data(), sizeof(double) * n4, hipMemcpyHostToDevice, s));
     const int KP = D->kpad;
-    std::vector<double> V((size_t)KP * KP, 0.0), kv((size_t)KP, 0.0);
-    auto g = [&](int p, int q, int r, int t) { return two[(((size_t)p * n + q) * n + r) * n + t]; };
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j <= i; ++j) {
-            const int p = i * (i + 1) / 2 + j;
-            double kk = one[(size_t)i * n + j];
-            for (int r = 0; r < n; ++r) kk -= 0.5 * g(i, r, r, j);  // (ir|rj) = <ir|rj>
-            kv[(size_t)p] = kk;
-            for (int k = 0; k < n; ++k)
-                for (int l = 0; l <= k; ++l) V[(size_t)p * KP + k * (k + 1) / 2 + l] = g(i, k, j, l);  // (ij|kl)
-        }
+    std::vector<double> V, kv;
+    pair_integrals(n, one.data(), two.data(), KP, V, kv);
     std::vector<float> V32(V.begin(), V.end()), kv32(kv.begin(), kv.end());
     D->V64.alloc(V.size());
     D->k64.alloc(kv.size());

@@ -178,6 +178,9 @@ struct pyci_op {
     // filled only on demand (exports, element queries)
     int storage = PYCI_STORAGE_CSR;
     std::unique_ptr<pe::DirectOp> direct;
+    // beta-string count when the rows are the full FullCI space in colex order with
+    // nup == ndn (the alpha <-> beta exchange is then J = a nb + b -> b nb + a), else 0
+    pe::i64 swap_nb = 0;
 };
 
 namespace pe {
@@ -315,6 +318,19 @@ template <typename T>
 void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, hipEvent_t *ev = nullptr);
 void direct_bench_parts(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *ms, double *bytes, double *flops);
 void direct_ensure_csr(pyci_op *op);
+// helpers shared with the batched solver (direct.hip)
+std::vector<u64> colex_strings(int n, int k);
+i64 rank_host(u64 det, const std::vector<i64> &binom);
+std::vector<int2> string_table(int n, const std::vector<u64> &S, int ne, const std::vector<i64> &binom);
+void pair_integrals(int n, const double *one, const double *two, int kp, std::vector<double> &V,
+                    std::vector<double> &kv);
+void mf_diag_launch(i64 nd, i64 NB, const u64 *sa, const u64 *sb, HamView H, double *diag, hipStream_t s);
+void mf_wallclock_launch(unsigned long long *t, hipStream_t s);
+// Batched solver (batch.hip)
+void batch_solve(pyci_ctx *ctx, i64 nprob, const i64 *nbasis, const i64 *nup, const i64 *ndn, const double *ecore,
+                 const double *const *one_mo, const double *const *two_mo, i64 nroot, double tol, int precision,
+                 i64 maxiter, double *evals, double *latency_ms, i64 *iters, i64 *nsigma, int *converged,
+                 pyci_batch_stats *stats);
 
 // Learned selector (mlp.hip)
 void mlp_init(pyci_mlp *m, int nfeat, u64 seed);

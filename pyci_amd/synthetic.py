@@ -65,3 +65,21 @@ def spin_orbital(one_mo, two_mo):
         for s2 in (0, n):
             g[s1:s1 + n, s2:s2 + n, s1:s1 + n, s2:s2 + n] = two_mo
     return h, g
+
+
+def batch_problems(count=64, seed=4000, nchol=32):
+    """The uneven-batch workload: `count` FullCI problems with nbasis ~ U{8..12} and
+    m ~ U{3..6} electrons of each spin, drawn (with lambda ~ U[0, 1]) from
+    default_rng(seed) in the order nbasis, m, lambda per problem; integrals
+    (1 - lambda) A + lambda B, with A and B the recipe above (h_ii = -2 + 0.25 i,
+    `nchol` vectors) at that nbasis with seeds 2000 + k and 3000 + k."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        nb = int(rng.integers(8, 13))
+        m = int(rng.integers(3, 7))
+        lam = float(rng.random())
+        _, oa, ta = seeded_integrals(nb, nchol, 2000 + k)
+        _, ob, tb = seeded_integrals(nb, nchol, 3000 + k)
+        out.append(dict(nbasis=nb, nocc=m, lam=lam, one=(1 - lam) * oa + lam * ob, two=(1 - lam) * ta + lam * tb))
+    return out
