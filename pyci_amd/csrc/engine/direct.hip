@@ -35,14 +35,43 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
+// Determinant of this thread. TILED: the 256 threads of a block cover a 16 x 16
+// tile (alpha rank x beta rank) of the string matrix, and tiles of one column range
+// run on one XCD (blocks go round-robin over the 8 XCDs): the alpha gathers, which
+// walk columns of x, then find them in that XCD's L2. Otherwise 256 consecutive
+// determinants per block.
+template <bool TILED>
+__device__ inline bool det_of(i64 nd, i64 NA, i64 NB, i64 &Ja, i64 &Jb) {
+    if constexpr (TILED) {
+        const i64 nta = (NA + 15) / 16, ntb = (NB + 15) / 16, slots = (ntb + 7) / 8;
+        const i64 b = blockIdx.x, xcd = b % 8, i = b / 8;
+        const i64 tcol = xcd + 8 * (i % slots), trow = i / slots;
+        if (tcol >= ntb || trow >= nta) return false;
+        Ja = trow * 16 + (threadIdx.x >> 4);
+        Jb = tcol * 16 + (threadIdx.x & 15);
+        return Ja < NA && Jb < NB;
+    } else {
+        const i64 J = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+        if (J >= nd) return false;
+        Ja = J / NB;
+        Jb = J - Ja * NB;
+        return true;
+    }
+}
+
+static inline unsigned tiled_grid(i64 NA, i64 NB) {
+    const i64 nta = (NA + 15) / 16, ntb = (NB + 15) / 16;
+    return (unsigned)(8 * ((ntb + 7) / 8) * nta);
+}
+
 // D pass: one thread per determinant J; NV >= nvec vectors (row-major X[J][v]).
-template <int NV, typename T>
+template <int NV, typename T, bool TILED>
 __global__ void __launch_bounds__(256) k_mf_d(i64 nd, i64 NB, int npair, int nea, int neb, const int2 *__restrict__ ta,
                                               const int2 *__restrict__ tb, int nvec, const T *__restrict__ X,
                                               T *__restrict__ W, i64 ldw) {
-    const i64 J = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (J >= nd) return;
-    const i64 Ja = J / NB, Jb = J - Ja * NB;
+    i64 Ja, Jb;
+    if (!det_of<TILED>(nd, nd / NB, NB, Ja, Jb)) return;
+    const i64 J = Ja * NB + Jb;
     const int2 *pa = ta + Ja * nea, *pb = tb + Jb * neb;
     int ia = 0, ib = 0;
     int2 ea = pa[0], eb = pb[0];
@@ -128,14 +157,64 @@ __global__ void __launch_bounds__(256) k_mf_gemm(i64 ncol, const T *__restrict__
     }
 }
 
+// Product pass, second form: each wave owns 16 columns and first loads all of its
+// B fragments (KP / 4 per lane) so that every load is in flight at once; V (and k)
+// sit in LDS when VLDS (kpad^2 values, row stride kpad + 1 against bank conflicts),
+// else they are read through the caches. 256 threads = 64 columns per block.
+template <int MT, typename T, bool VLDS>
+__global__ void __launch_bounds__(256) k_mf_gemm2(i64 ncol, const T *__restrict__ V, const T *__restrict__ kv,
+                                                  const T *__restrict__ X, T *W, i64 ldw) {
+    constexpr int KP = MT * 16, NS = KP / 4, LS = KP + 1;
+    extern __shared__ unsigned char smem_raw[];
+    T *vs = reinterpret_cast<T *>(smem_raw);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    if constexpr (VLDS) {
+        // V once per block; the block then walks column tiles of 64 (persistent grid)
+        for (int i = threadIdx.x; i < KP * KP; i += 256) vs[(i / KP) * LS + (i % KP)] = V[i];
+        __syncthreads();
+    }
+    typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type acc_t;
+    const i64 ntiles = (ncol + 63) / 64;
+    for (i64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const i64 c = tile * 64 + wave * 16 + lr;
+        const bool live = c < ncol;
+        T b[NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) b[q] = live ? W[(i64)(4 * q + lk) * ldw + c] : (T)0;
+        acc_t acc[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) acc[m] = acc_t{};
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                const T a = VLDS ? vs[(16 * m + lr) * LS + 4 * q + lk] : V[(16 * m + lr) * KP + 4 * q + lk];
+                if constexpr (sizeof(T) == 8) acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[q], acc[m], 0, 0, 0);
+                else acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[q], acc[m], 0, 0, 0);
+            }
+        }
+        if (live) {
+            const T x = X[c];
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int p = (sizeof(T) == 8) ? 16 * m + lk + 4 * r : 16 * m + 4 * lk + r;
+                    W[(i64)p * ldw + c] = (T)0.5 * acc[m][r] + kv[p] * x;
+                }
+        }
+    }
+}
+
 // Scatter pass: Y[I][v] = sum over alpha and beta table entries of I.
-template <int NV, typename T>
+template <int NV, typename T, bool TILED>
 __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int neb, const int2 *__restrict__ ta,
                                                   const int2 *__restrict__ tb, int nvec, const T *__restrict__ W,
                                                   i64 ldw, T *__restrict__ Y) {
-    const i64 I = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= nd) return;
-    const i64 Ia = I / NB, Ib = I - Ia * NB;
+    i64 Ia, Ib;
+    if (!det_of<TILED>(nd, nd / NB, NB, Ia, Ib)) return;
+    const i64 I = Ia * NB + Ib;
     T acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = (T)0;
@@ -405,7 +484,16 @@ float *direct_workspace<float>(DirectOp *D, i64 ldw, hipStream_t s) {
 template <int MT, typename T>
 static void launch_gemm(i64 ncol, const T *V, const T *kv, const T *X, T *W, i64 ldw, hipStream_t s) {
     // only the column tiles that hold the ncol live columns (ldw >= their end)
-    k_mf_gemm<MT, T><<<(unsigned)((ncol + 127) / 128), 256, 0, s>>>(ncol, V, kv, X, W, ldw);
+    constexpr size_t lds = (size_t)(MT * 16) * (MT * 16 + 1) * sizeof(T);
+    const i64 ntiles = (ncol + 63) / 64;
+    if (lds <= 65536) {
+        // persistent: as many blocks as fit on the 256 CUs (160 KB of LDS each), V loaded once per block
+        const i64 per_cu = std::max<i64>(1, std::min<i64>(8, (160 * 1024) / (i64)lds));
+        const unsigned grid = (unsigned)std::min<i64>(ntiles, 256 * per_cu);
+        k_mf_gemm2<MT, T, true><<<grid, 256, lds, s>>>(ncol, V, kv, X, W, ldw);
+    } else {
+        k_mf_gemm2<MT, T, false><<<(unsigned)ntiles, 256, 0, s>>>(ncol, V, kv, X, W, ldw);
+    }
     PE_LAUNCH_CHECK();
 }
 
@@ -413,9 +501,15 @@ static void launch_gemm(i64 ncol, const T *V, const T *kv, const T *X, T *W, i64
 template <int NV, typename T>
 static void launch_passes(const DirectOp *D, i64 nd, int nvec, const T *X, T *W, i64 ldw, T *Y, const T *V,
                           const T *kv, hipStream_t s, hipEvent_t *ev) {
-    const unsigned g = grid_for(nd, 256);
+    const bool tiled = (D->nbs + 15) / 16 >= 16;  // enough column tiles to spread over the XCDs
+    const unsigned g = tiled ? tiled_grid(D->na, D->nbs) : grid_for(nd, 256);
     if (ev) PE_HIP(hipEventRecord(ev[0], s));
-    k_mf_d<NV, T><<<g, 256, 0, s>>>(nd, D->nbs, D->npair, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, X, W, ldw);
+    if (tiled)
+        k_mf_d<NV, T, true><<<g, 256, 0, s>>>(nd, D->nbs, D->npair, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, X,
+                                              W, ldw);
+    else
+        k_mf_d<NV, T, false><<<g, 256, 0, s>>>(nd, D->nbs, D->npair, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, X,
+                                               W, ldw);
     PE_LAUNCH_CHECK();
     if (ev) PE_HIP(hipEventRecord(ev[1], s));
     const i64 ncol = nd * nvec;
@@ -431,7 +525,12 @@ static void launch_passes(const DirectOp *D, i64 nd, int nvec, const T *X, T *W,
         default: throw EngineError(PYCI_EUNSUPPORTED, "pair dimension too large for the matrix-free operator");
     }
     if (ev) PE_HIP(hipEventRecord(ev[2], s));
-    k_mf_sigma<NV, T><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, W, ldw, Y);
+    if (tiled)
+        k_mf_sigma<NV, T, true><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, W, ldw,
+                                                  Y);
+    else
+        k_mf_sigma<NV, T, false><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, W, ldw,
+                                                   Y);
     PE_LAUNCH_CHECK();
     if (ev) PE_HIP(hipEventRecord(ev[3], s));
 }
