@@ -416,7 +416,7 @@ def main():
     tparts = parts(op, nroot, 20 if nroot == 1 else 3, fp32)
     L.pyci_op_destroy(op)
     names = ["k_spmv<4> (CSR sigma)"] if storage == "csr" else [
-        "k_mf_d (gather pass)", "k_mf_gemm (pair-integral product, MFMA)", "k_mf_sigma (scatter pass)"]
+        "k_mf_dprod (gather + pair-integral product, MFMA)", "k_mf_sigma (scatter pass)"]
     for nm, v in zip(names, tparts):
         log(f"{storage} sigma part {nm}: {v['ms']:.3f} ms, {v['bytes'] / v['ms'] / 1e6:.0f} GB/s, "
             f"{v['flops'] / v['ms'] / 1e9:.1f} TFLOP/s")
@@ -445,7 +445,7 @@ def main():
     pk = "f32" if fp32 else "f64"
     ridge = MFMA_PEAK_TFLOPS[pk] * 1e12 / (HBM_PEAK_GBPS * 1e9)
     kname = names[dom]
-    slug = "sigma_csr" if storage == "csr" else ["mf_gather", "mf_product", "mf_scatter"][dom]
+    slug = "sigma_csr" if storage == "csr" else ["mf_dprod", "mf_scatter"][dom]
     traffic = measured_traffic(f"{slug}_{wl['name']}_{pk}_hbm_traffic.md")
     if dk["flops"] / dk["bytes"] > ridge:
         achieved = dk["flops"] / (dk["ms"] * 1e-3) / 1e12

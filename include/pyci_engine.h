@@ -145,7 +145,7 @@ int pyci_op_matvec_device(pyci_op *op, int64_t nvec, const double *x_dev, double
  * algorithmic bytes of one launch. fp32 != 0 times the FP32-value kernel. */
 int pyci_op_bench_sigma(pyci_op *op, int64_t nvec, int64_t nrep, int fp32, float *avg_ms, double *bytes);
 /* Per-kernel view of the same measurement: *nparts kernels per sigma (1 for the
- * CSR; 3 for the matrix-free operator: gather pass, pair-integral product, scatter
+ * CSR; 2 for the matrix-free operator: fused gather + pair-integral product, scatter
  * pass), each with its average duration (HIP events between the kernels of each
  * of nrep sigmas), algorithmic bytes and flops per launch. Arrays hold >= 4. */
 int pyci_op_bench_sigma_parts(pyci_op *op, int64_t nvec, int64_t nrep, int fp32, int *nparts, float *avg_ms,

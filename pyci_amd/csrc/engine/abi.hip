@@ -491,7 +491,7 @@ int pyci_op_bench_sigma_parts(pyci_op *op, int64_t nvec, int64_t nrep, int fp32,
         PE_REQUIRE(nvec >= 1 && nrep >= 1, PYCI_EINVAL, "nvec and nrep must be >= 1");
         set_device(op->ctx);
         if (op->storage == PYCI_STORAGE_DIRECT) {
-            *nparts = 3;
+            *nparts = 2;
             direct_bench_parts(op, nvec, nrep, fp32, avg_ms, bytes, flops);
         } else {
             *nparts = 1;

@@ -64,154 +64,244 @@ static inline unsigned tiled_grid(i64 NA, i64 NB) {
     return (unsigned)(8 * ((ntb + 7) / 8) * nta);
 }
 
-// D pass: one thread per determinant J; NV >= nvec vectors (row-major X[J][v]).
-template <int NV, typename T, bool TILED>
-__global__ void __launch_bounds__(256) k_mf_d(i64 nd, i64 NB, int npair, int nea, int neb, const int2 *__restrict__ ta,
-                                              const int2 *__restrict__ tb, int nvec, const T *__restrict__ X,
-                                              T *__restrict__ W, i64 ldw) {
-    i64 Ja, Jb;
-    if (!det_of<TILED>(nd, nd / NB, NB, Ja, Jb)) return;
-    const i64 J = Ja * NB + Jb;
-    const int2 *pa = ta + Ja * nea, *pb = tb + Jb * neb;
-    int ia = 0, ib = 0;
-    int2 ea = pa[0], eb = pb[0];
-    for (int q = 0; q < npair; ++q) {
-        T acc[NV];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] = (T)0;
-        if (ia < nea && (ea.y & 0xffff) == q) {
-            const T s = (ea.y >> 16) ? (T)-1 : (T)1;
-            const T *x = X + ((i64)ea.x * NB + Jb) * nvec;
-#pragma unroll
-            for (int v = 0; v < NV; ++v)
-                if (v < nvec) acc[v] += s * x[v];
-            if (++ia < nea) ea = pa[ia];
-        }
-        if (ib < neb && (eb.y & 0xffff) == q) {
-            const T s = (eb.y >> 16) ? (T)-1 : (T)1;
-            const T *x = X + (Ja * NB + eb.x) * nvec;
-#pragma unroll
-            for (int v = 0; v < NV; ++v)
-                if (v < nvec) acc[v] += s * x[v];
-            if (++ib < neb) eb = pb[ib];
-        }
-        T *w = W + (i64)q * ldw + J * nvec;
-#pragma unroll
-        for (int v = 0; v < NV; ++v)
-            if (v < nvec) w[v] = acc[v];
-    }
+__device__ inline f64x4 mfma16(double a, double b, f64x4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ inline f32x4 mfma16(float a, float b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+// An empty asm statement that takes v as an operand: the value must exist here,
+// so the load that produces it is issued unconditionally where the source has it
+// (instead of being sunk into a branch of its own that waits for it alone).
+__device__ inline void pin(int &v) { asm volatile("" : "+v"(v)); }
+__device__ inline void pin(float &v) { asm volatile("" : "+v"(v)); }
+__device__ inline void pin(double &v) { asm volatile("" : "+v"(v)); }
+
+// Workspace layout: column c = J * nvec + v (determinant J, vector v) of pair row
+// p sits at W[(c / 64) * KP * 64 + p * 64 + c % 64]: the rows of one 64-column tile
+// form one contiguous block of KP * 64 values, which the product pass writes and
+// the scatter pass reads in 512-byte (FP64) runs.
+__host__ __device__ inline i64 wt_at(i64 c, int p, int KP) { return (c >> 6) * (i64)KP * 64 + p * 64 + (c & 63); }
+
+// Gather + product pass, fused. Per 64-column tile:
+//   D[q][c] = sum over the alpha and beta table entries of J with pair q of s X[K][v]
+//   E[p][c] = 1/2 sum_q V[p][q] D[q][c] + k[p] X[c]        -> W (pair rows; pad rows are
+//                                                             never read)
+// 768 threads: waves 4..11 gather, waves 0..3 multiply; the grid is persistent
+// (tile = blockIdx.x + step * gridDim.x) and D never reaches HBM. With two LDS
+// buffers (SB false) the gather of tile t+1 runs beside the product of tile t, one
+// barrier per tile. With one buffer (SB true, where two do not fit beside V) the
+// gather waves add tile t into the buffer between two barriers and load the X
+// values of tile t+1 into registers while the product of tile t runs.
+// Gather: lane = column; wave 4 + g takes table entries g, g + 8, ... (PE slots,
+// 8 PE >= nea + neb) of the merged table (alpha rows, then beta rows, then one
+// sentinel entry that every empty slot reads: it adds into the last buffer row,
+// which nothing reads), loading the entries of its next tile while X values are in
+// flight. Indices are 32-bit (nd nvec < 2^31). A (pair, column) cell receives at
+// most one alpha and one beta term (a string's entries have distinct pairs), added
+// onto zero, so the LDS atomic adds give the same bits in any order. D is held in
+// FP64 for both value types (LDS FP64 adds are fast where FP32 ones are not; an
+// FP32 D value is the sum of two FP32 terms, rounded once).
+// Product: wave w owns columns 16 w .. 16 w + 15 of the tile and the row tiles; the
+// k loop is unrolled, with the A and B fragments of step q + 1 loaded before the
+// MFMAs of step q; each lane clears the D values it has read, so the buffer is zero
+// again when it is next filled. MFMA 16x16x4: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; accumulator r of lane l is D[4 (l / 16) + r][l % 16] for the
+// FP32 instruction and D[l / 16 + 4 r][l % 16] for the FP64 one (lane layouts checked
+// on the device). V (kpad x kpad, zero padded) is read from LDS (VM_FULL, row
+// stride VC + 1) or through the caches (VM_GLOBAL). For kpad >= 96 the pair count
+// is fixed by kpad (91, 105, 120 for nbasis 13, 14, 15), so the k extent VC, the
+// buffer rows and the row tiles stop at the last pair. Buffer row stride ld = 80
+// puts the four 16-lane groups of a B read on distinct LDS banks (64 where 80 does
+// not fit).
+constexpr int kDprodThreads = 768;
+enum { VM_GLOBAL = 0, VM_FULL = 1 };
+
+// pair count implied by a pair dimension of kpad = 16 MT >= 96 (else 0: not fixed)
+__host__ __device__ constexpr int fixed_npair(int MT) { return MT == 6 ? 91 : MT == 7 ? 105 : MT == 8 ? 120 : 0; }
+// k extent of the product (a multiple of 4); the sink row of the gather buffers is row VC
+__host__ __device__ constexpr int dprod_vc(int MT) {
+    return fixed_npair(MT) ? 4 * ((fixed_npair(MT) + 3) / 4) : 16 * MT;
 }
 
-// Product pass, in place on W (kpad rows, ldw columns, ldw % 128 == 0):
-//   W[p][c] = 1/2 sum_q V[p][q] W[q][c] + k[p] X[c]      (X[c] = 0 for c >= ncol)
-// V is kpad x kpad (zero padded), MT = kpad / 16 row tiles. 256 threads: each
-// wave owns 32 columns (two 16-column tiles) and all MT row tiles; every column
-// is read completely before the wave writes it. MFMA 16x16x4: lane l supplies
-// A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator r of lane l is
-// D[4 (l / 16) + r][l % 16] for the FP32 instruction and D[l / 16 + 4 r][l % 16]
-// for the FP64 one (lane layouts checked on the device).
-template <int MT, typename T>
-__global__ void __launch_bounds__(256) k_mf_gemm(i64 ncol, const T *__restrict__ V, const T *__restrict__ kv,
-                                                 const T *__restrict__ X, T *W, i64 ldw) {
-    constexpr int KP = MT * 16;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
-    const i64 c0 = (i64)blockIdx.x * 128 + wave * 32;
-    typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type acc_t;
-    acc_t acc[MT][2];
-#pragma unroll
-    for (int m = 0; m < MT; ++m) {
-        acc[m][0] = acc_t{};
-        acc[m][1] = acc_t{};
-    }
-    // columns at or beyond ncol are read as zero: they stay zero in every row, so
-    // the zero pad rows never meet a non-finite value
-    const bool live0 = c0 + lr < ncol, live1 = c0 + 16 + lr < ncol;
-#pragma unroll 2
-    for (int k0 = 0; k0 < KP; k0 += 4) {
-        const T b0 = live0 ? W[(i64)(k0 + lk) * ldw + c0 + lr] : (T)0;
-        const T b1 = live1 ? W[(i64)(k0 + lk) * ldw + c0 + 16 + lr] : (T)0;
-#pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            const T a = V[(16 * m + lr) * KP + k0 + lk];
-            if constexpr (sizeof(T) == 8) {
-                acc[m][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[m][0], 0, 0, 0);
-                acc[m][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[m][1], 0, 0, 0);
-            } else {
-                acc[m][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[m][0], 0, 0, 0);
-                acc[m][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[m][1], 0, 0, 0);
-            }
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const i64 c = c0 + 16 * t + lr;
-        const T x = (c < ncol) ? X[c] : (T)0;
-#pragma unroll
-        for (int m = 0; m < MT; ++m)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int p = (sizeof(T) == 8) ? 16 * m + lk + 4 * r : 16 * m + 4 * lk + r;
-                W[(i64)p * ldw + c] = (T)0.5 * acc[m][t][r] + kv[p] * x;
-            }
-    }
-}
+struct DprodArgs {
+    int ncol, NB, nvec, npair, nea, ntot;
+    int offb;  // first beta row of the merged table, NA * nea
+    int sent;  // index of the sentinel entry
+    int ld;
+};
 
-// Product pass, second form: each wave owns 16 columns and first loads all of its
-// B fragments (KP / 4 per lane) so that every load is in flight at once; V (and k)
-// sit in LDS when VLDS (kpad^2 values, row stride kpad + 1 against bank conflicts),
-// else they are read through the caches. 256 threads = 64 columns per block.
-template <int MT, typename T, bool VLDS>
-__global__ void __launch_bounds__(256) k_mf_gemm2(i64 ncol, const T *__restrict__ V, const T *__restrict__ kv,
-                                                  const T *__restrict__ X, T *W, i64 ldw) {
-    constexpr int KP = MT * 16, NS = KP / 4, LS = KP + 1;
+template <int MT, typename T, bool SB, int VM, int PE>
+__global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__restrict__ tab, const T *__restrict__ X,
+                                                  const T *__restrict__ V, const T *__restrict__ kv,
+                                                  T *__restrict__ W) {
+    constexpr int KP = MT * 16, NT = kDprodThreads;
+    constexpr int NP = fixed_npair(MT);             // 0: pair count known only at run time
+    constexpr int VC = dprod_vc(MT), NQ = VC / 4;   // k extent and k steps
+    constexpr int MTL = NP ? (NP + 15) / 16 : MT;   // row tiles that hold a pair
+    constexpr int BR = VC + 1, LS = VC + 1, NBUF = SB ? 1 : 2;
+    static_assert(!(SB && VM == VM_GLOBAL), "one buffer only with V in LDS");
     extern __shared__ unsigned char smem_raw[];
-    T *vs = reinterpret_cast<T *>(smem_raw);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
-    if constexpr (VLDS) {
-        // V once per block; the block then walks column tiles of 64 (persistent grid)
-        for (int i = threadIdx.x; i < KP * KP; i += 256) vs[(i / KP) * LS + (i % KP)] = V[i];
-        __syncthreads();
-    }
-    typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type acc_t;
-    const i64 ntiles = (ncol + 63) / 64;
-    for (i64 tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const i64 c = tile * 64 + wave * 16 + lr;
-        const bool live = c < ncol;
-        T b[NS];
+    double *buf0 = reinterpret_cast<double *>(smem_raw);     // NBUF buffers of BR x ld
+    T *ks = reinterpret_cast<T *>(buf0 + NBUF * BR * A.ld);  // k[p], kpad values
+    T *vs = ks + KP;                                         // V rows 0..KP-1, columns 0..VC-1
+    const int ld = A.ld;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ntiles = (A.ncol + 63) / 64;
+    const int nmine = (ntiles > (int)blockIdx.x) ? (ntiles - 1 - (int)blockIdx.x) / (int)gridDim.x + 1 : 0;
+    for (int i = threadIdx.x; i < NBUF * BR * ld; i += NT) buf0[i] = 0.0;
+    for (int i = threadIdx.x; i < KP; i += NT) ks[i] = kv[i];
+    if constexpr (VM == VM_FULL)
+        for (int i = threadIdx.x; i < KP * VC; i += NT) vs[(i / VC) * LS + (i % VC)] = V[(i / VC) * KP + (i % VC)];
+    // The two roles run separate loops with the same barrier count, so that neither
+    // role's registers are live in the other.
+    if (wave >= 4) {
+        const int g = wave - 4, sa = A.NB * A.nvec;
+        // loads the table entries of the tile of `step` into nx and its X bases: the X
+        // index of an alpha entry is en.x * sa + na, of a beta entry en.x * nvec + nb
+        auto fetch = [&](int step, int2 *nx, int &na, int &nb) {
+            const int c0 = ((int)blockIdx.x + step * (int)gridDim.x) * 64 + lane;
+            const bool live = step < nmine && c0 < A.ncol;
+            const unsigned c = live ? (unsigned)c0 : 0u;
+            const unsigned J = c / (unsigned)A.nvec, v = c - J * (unsigned)A.nvec;
+            const unsigned Ja = J / (unsigned)A.NB, Jb = J - Ja * (unsigned)A.NB;
+            const int ra = (int)(Ja * (unsigned)A.nea), rb = A.offb + (int)(Jb * (unsigned)(A.ntot - A.nea)) - A.nea;
+            na = (int)(Jb * (unsigned)A.nvec + v);
+            nb = (int)(Ja * (unsigned)sa + v);
 #pragma unroll
-        for (int q = 0; q < NS; ++q) b[q] = live ? W[(i64)(4 * q + lk) * ldw + c] : (T)0;
-        acc_t acc[MT];
+            for (int u = 0; u < PE; ++u) {
+                const int t = g + 8 * u;
+                const int at = (t < A.nea) ? ra + t : rb + t;
+                nx[u] = tab[(live && t < A.ntot) ? at : A.sent];
+            }
+        };
+        auto xload = [&](const int2 *en, int xa, int xb, T *val) {
 #pragma unroll
-        for (int m = 0; m < MT; ++m) acc[m] = acc_t{};
+            for (int u = 0; u < PE; ++u) {
+                const int t = g + 8 * u;
+                val[u] = X[(t < A.nea) ? en[u].x * sa + xa : en[u].x * A.nvec + xb];
+            }
+        };
+        auto add = [&](const int2 *en, T *val, double *buf) {
 #pragma unroll
-        for (int q = 0; q < NS; ++q) {
+            for (int u = 0; u < PE; ++u) pin(val[u]);
 #pragma unroll
-            for (int m = 0; m < MT; ++m) {
-                const T a = VLDS ? vs[(16 * m + lr) * LS + 4 * q + lk] : V[(16 * m + lr) * KP + 4 * q + lk];
-                if constexpr (sizeof(T) == 8) acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[q], acc[m], 0, 0, 0);
-                else acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[q], acc[m], 0, 0, 0);
+            for (int u = 0; u < PE; ++u) {
+                const double x = (double)val[u];
+                atomicAdd(buf + lane + (en[u].y & 0xffff) * ld, (en[u].y >> 16) ? -x : x);
+            }
+        };
+        int2 en[PE], nx[PE];
+        T val[PE];
+        int xa, xb, na, nb;
+        fetch(0, en, xa, xb);
+        if constexpr (!SB) {
+            __syncthreads();
+            for (int step = 0; step <= nmine; ++step) {
+                if (step < nmine) {
+                    xload(en, xa, xb, val);
+                    fetch(step + 1, nx, na, nb);  // in flight beside the X loads
+                    add(en, val, buf0 + (step & 1) * BR * ld);
+#pragma unroll
+                    for (int u = 0; u < PE; ++u) en[u] = nx[u];
+                    xa = na;
+                    xb = nb;
+                }
+                __syncthreads();
+            }
+        } else {
+            xload(en, xa, xb, val);
+            fetch(1, nx, na, nb);
+            __syncthreads();
+            for (int step = 0; step < nmine; ++step) {
+                __syncthreads();  // the product waves are done with the buffer
+                add(en, val, buf0);
+                __syncthreads();  // the buffer holds D of this tile
+#pragma unroll
+                for (int u = 0; u < PE; ++u) en[u] = nx[u];
+                xload(en, na, nb, val);  // the next tile's X values, in flight during the product
+                fetch(step + 2, nx, na, nb);
             }
         }
-        if (live) {
-            const T x = X[c];
+    } else {
+        const int lr = lane & 15, lk = lane >> 4;
+        typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type acc_t;
+        auto aload = [&](int m, int q) -> T {
+            if constexpr (VM == VM_FULL) return vs[(16 * m + lr) * LS + 4 * q + lk];
+            else return V[(16 * m + lr) * KP + 4 * q + lk];
+        };
+        // X[c] of the tile multiplied next, loaded one tile ahead
+        T x = X[min((int)blockIdx.x * 64 + 16 * wave + lr, A.ncol - 1)];
+        auto product = [&](int tile, double *buf) {
+            const int c = tile * 64 + 16 * wave + lr;
+            double *bb = buf + 16 * wave + lr;
+            acc_t acc[MT];
 #pragma unroll
-            for (int m = 0; m < MT; ++m)
+            for (int m = 0; m < MT; ++m) acc[m] = acc_t{};
+            if constexpr (VM == VM_GLOBAL) {
+                // V through the caches (nbasis 15 in FP64 only): one k step at a time
+#pragma unroll 1
+                for (int q = 0; q < NQ; ++q) {
+                    const T bq = (T)bb[(4 * q + lk) * ld];
+                    bb[(4 * q + lk) * ld] = 0.0;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int p = (sizeof(T) == 8) ? 16 * m + lk + 4 * r : 16 * m + 4 * lk + r;
-                    W[(i64)p * ldw + c] = (T)0.5 * acc[m][r] + kv[p] * x;
+                    for (int m = 0; m < MTL; ++m) acc[m] = mfma16(aload(m, q), bq, acc[m]);
                 }
+            } else {
+                T ac[MTL], an[MTL], bc, bn;
+                bc = (T)bb[lk * ld];
+                bb[lk * ld] = 0.0;
+#pragma unroll
+                for (int m = 0; m < MTL; ++m) ac[m] = aload(m, 0);
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    if (q + 1 < NQ) {
+                        bn = (T)bb[(4 * q + 4 + lk) * ld];
+                        bb[(4 * q + 4 + lk) * ld] = 0.0;
+#pragma unroll
+                        for (int m = 0; m < MTL; ++m) an[m] = aload(m, q + 1);
+                    }
+#pragma unroll
+                    for (int m = 0; m < MTL; ++m) acc[m] = mfma16(ac[m], bc, acc[m]);
+                    // keeps the fragment loads of later steps from being hoisted above
+                    asm volatile("" ::: "memory");
+#pragma unroll
+                    for (int m = 0; m < MTL; ++m) ac[m] = an[m];
+                    bc = bn;
+                }
+            }
+            if (c < A.ncol) {
+                T *w = W + (i64)tile * KP * 64 + 16 * wave + lr;
+#pragma unroll
+                for (int m = 0; m < MTL; ++m)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int p = (sizeof(T) == 8) ? 16 * m + lk + 4 * r : 16 * m + 4 * lk + r;
+                        w[p * 64] = (T)0.5 * acc[m][r] + ks[p] * x;
+                    }
+            }
+            x = X[min(c + 64 * (int)gridDim.x, A.ncol - 1)];
+        };
+        if constexpr (!SB) {
+            __syncthreads();
+            for (int step = 0; step <= nmine; ++step) {
+                if (step >= 1)
+                    product((int)blockIdx.x + (step - 1) * (int)gridDim.x, buf0 + ((step - 1) & 1) * BR * ld);
+                __syncthreads();
+            }
+        } else {
+            __syncthreads();
+            for (int step = 0; step < nmine; ++step) {
+                __syncthreads();
+                __syncthreads();
+                product((int)blockIdx.x + step * (int)gridDim.x, buf0);
+            }
         }
     }
 }
 
-// Scatter pass: Y[I][v] = sum over alpha and beta table entries of I.
+// Scatter pass: Y[I][v] = sum over the alpha and beta table entries of I of s E[pair][K v].
 template <int NV, typename T, bool TILED>
-__global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int neb, const int2 *__restrict__ ta,
-                                                  const int2 *__restrict__ tb, int nvec, const T *__restrict__ W,
-                                                  i64 ldw, T *__restrict__ Y) {
+__global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int neb, int KP,
+                                                  const int2 *__restrict__ ta, const int2 *__restrict__ tb, int nvec,
+                                                  const T *__restrict__ W, T *__restrict__ Y) {
     i64 Ia, Ib;
     if (!det_of<TILED>(nd, nd / NB, NB, Ia, Ib)) return;
     const i64 I = Ia * NB + Ib;
@@ -222,19 +312,19 @@ __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int n
     for (int t = 0; t < nea; ++t) {
         const int2 e = pa[t];
         const T s = (e.y >> 16) ? (T)-1 : (T)1;
-        const T *w = W + (i64)(e.y & 0xffff) * ldw + ((i64)e.x * NB + Ib) * nvec;
+        const i64 c = ((i64)e.x * NB + Ib) * nvec;
 #pragma unroll
         for (int v = 0; v < NV; ++v)
-            if (v < nvec) acc[v] += s * w[v];
+            if (v < nvec) acc[v] += s * W[wt_at(c + v, e.y & 0xffff, KP)];
     }
     const int2 *pb = tb + Ib * neb;
     for (int t = 0; t < neb; ++t) {
         const int2 e = pb[t];
         const T s = (e.y >> 16) ? (T)-1 : (T)1;
-        const T *w = W + (i64)(e.y & 0xffff) * ldw + (Ia * NB + e.x) * nvec;
+        const i64 c = (Ia * NB + e.x) * nvec;
 #pragma unroll
         for (int v = 0; v < NV; ++v)
-            if (v < nvec) acc[v] += s * w[v];
+            if (v < nvec) acc[v] += s * W[wt_at(c + v, e.y & 0xffff, KP)];
     }
     T *y = Y + I * nvec;
 #pragma unroll
@@ -413,6 +503,11 @@ void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn) {
     const std::vector<int2> ta = string_table(n, sa, D->nea, binom), tb = string_table(n, sb, D->neb, binom);
     D->tab_a.alloc(ta.size());
     D->tab_b.alloc(tb.size());
+    std::vector<int2> merged(ta);
+    merged.insert(merged.end(), tb.begin(), tb.end());
+    merged.push_back(int2{0, dprod_vc(D->kpad / 16)});  // sentinel: empty slots of the fused pass add into the sink row
+    D->tab.alloc(merged.size());
+    PE_HIP(hipMemcpyAsync(D->tab.p, merged.data(), sizeof(int2) * merged.size(), hipMemcpyHostToDevice, s));
     D->str_a.alloc(sa.size());
     D->str_b.alloc(sb.size());
     PE_HIP(hipMemcpyAsync(D->tab_a.p, ta.data(), sizeof(int2) * ta.size(), hipMemcpyHostToDevice, s));
@@ -458,81 +553,122 @@ void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn) {
     op->direct = std::move(D);
 }
 
+// Workspace of the E values: ceil(nd nvec / 64) tiles of kpad x 64 (tile-major);
+// grown on demand, never cleared (the scatter pass reads only what the product
+// pass of the same sigma wrote).
 template <typename T>
-static T *direct_workspace(DirectOp *D, i64 ldw, hipStream_t s);
+static T *direct_workspace(DirectOp *D, i64 cols);
 
 template <>
-double *direct_workspace<double>(DirectOp *D, i64 ldw, hipStream_t s) {
-    if (D->ldw64 < ldw) {
-        D->work64.alloc((size_t)D->kpad * ldw);
-        PE_HIP(hipMemsetAsync(D->work64.p, 0, sizeof(double) * D->kpad * ldw, s));  // pad rows / columns stay zero
-        D->ldw64 = ldw;
+double *direct_workspace<double>(DirectOp *D, i64 cols) {
+    if (D->ldw64 < cols) {
+        D->work64.alloc((size_t)D->kpad * cols);
+        D->ldw64 = cols;
     }
     return D->work64.p;
 }
 
 template <>
-float *direct_workspace<float>(DirectOp *D, i64 ldw, hipStream_t s) {
-    if (D->ldw32 < ldw) {
-        D->work32.alloc((size_t)D->kpad * ldw);
-        PE_HIP(hipMemsetAsync(D->work32.p, 0, sizeof(float) * D->kpad * ldw, s));
-        D->ldw32 = ldw;
+float *direct_workspace<float>(DirectOp *D, i64 cols) {
+    if (D->ldw32 < cols) {
+        D->work32.alloc((size_t)D->kpad * cols);
+        D->ldw32 = cols;
     }
     return D->work32.p;
 }
 
-template <int MT, typename T>
-static void launch_gemm(i64 ncol, const T *V, const T *kv, const T *X, T *W, i64 ldw, hipStream_t s) {
-    // only the column tiles that hold the ncol live columns (ldw >= their end)
-    constexpr size_t lds = (size_t)(MT * 16) * (MT * 16 + 1) * sizeof(T);
-    const i64 ntiles = (ncol + 63) / 64;
-    if (lds <= 65536) {
-        // persistent: as many blocks as fit on the 256 CUs (160 KB of LDS each), V loaded once per block
-        const i64 per_cu = std::max<i64>(1, std::min<i64>(8, (160 * 1024) / (i64)lds));
-        const unsigned grid = (unsigned)std::min<i64>(ntiles, 256 * per_cu);
-        k_mf_gemm2<MT, T, true><<<grid, 256, lds, s>>>(ncol, V, kv, X, W, ldw);
-    } else {
-        k_mf_gemm2<MT, T, false><<<(unsigned)ntiles, 256, 0, s>>>(ncol, V, kv, X, W, ldw);
+// LDS one block may use (160 KB on gfx950) and compute units (256), from the current device.
+static int device_attr(hipDeviceAttribute_t a) {
+    int dev = 0, v = 0;
+    PE_HIP(hipGetDevice(&dev));
+    PE_HIP(hipDeviceGetAttribute(&v, a, dev));
+    return v;
+}
+static size_t lds_per_block() { return (size_t)device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock); }
+
+template <int MT, typename T, bool SB, int VM, int PE>
+static void launch_dprod_v(const DirectOp *D, const DprodArgs &A, const T *X, const T *V, const T *kv, T *W,
+                           size_t lds, hipStream_t s) {
+    auto kern = k_mf_dprod<MT, T, SB, VM, PE>;
+    static bool attr = false;  // dynamic LDS above 64 KB
+    if (!attr) {
+        PE_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds_per_block()));
+        attr = true;
     }
+    // persistent grid: the blocks that are resident at once (registers and LDS)
+    int per_cu = 0;
+    PE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kDprodThreads, lds));
+    static const i64 ncu = device_attr(hipDeviceAttributeMultiprocessorCount);
+    const i64 ntiles = ((i64)A.ncol + 63) / 64;
+    const unsigned grid = (unsigned)std::max<i64>(1, std::min<i64>(ntiles, ncu * std::max(1, per_cu)));
+    kern<<<grid, kDprodThreads, lds, s>>>(A, D->tab.p, X, V, kv, W);
     PE_LAUNCH_CHECK();
 }
 
-// ev (optional): 4 events recorded before, between and after the three passes.
+// LDS plan of the fused pass for a pair dimension (gfx950: 160 KB per block): V in
+// LDS beside two buffers where that fits (row stride 80, else 64), else beside one
+// buffer, else V through the caches beside two buffers.
+template <int MT, typename T>
+struct DprodPlan {
+    static constexpr size_t KP = MT * 16, BR = dprod_vc(MT) + 1, cap = 160 * 1024;
+    static constexpr size_t vfull = KP * (dprod_vc(MT) + 1) * sizeof(T) + KP * sizeof(T);
+    static constexpr size_t bufs(int n, int ld) { return (size_t)n * BR * ld * sizeof(double); }
+    static constexpr int plan = vfull + bufs(2, 80) <= cap ? 0 : vfull + bufs(2, 64) <= cap ? 1
+                              : vfull + bufs(1, 80) <= cap ? 2 : vfull + bufs(1, 64) <= cap ? 3 : 4;
+    static constexpr bool SB = plan == 2 || plan == 3;
+    static constexpr int VM = plan == 4 ? VM_GLOBAL : VM_FULL;
+    static constexpr int ld = (plan == 1 || plan == 3) ? 64 : 80;
+    static constexpr size_t lds = plan == 4 ? KP * sizeof(T) + bufs(2, 80) : vfull + bufs(SB ? 1 : 2, ld);
+    static_assert(lds <= cap, "no LDS plan fits");
+};
+
+template <int MT, typename T>
+static void launch_dprod(const DirectOp *D, i64 nd, int nvec, const T *X, const T *V, const T *kv, T *W,
+                         hipStream_t s) {
+    typedef DprodPlan<MT, T> P;
+    static const size_t cap = lds_per_block();
+    const int ntot = D->nea + D->neb;
+    PE_REQUIRE(ntot <= 128, PYCI_EUNSUPPORTED, "too many single excitations per determinant");
+    PE_REQUIRE(nd * nvec < ((i64)1 << 31) - ((i64)1 << 20), PYCI_EUNSUPPORTED,
+               "too many columns for the matrix-free operator");
+    PE_REQUIRE(!fixed_npair(MT) || D->npair == fixed_npair(MT), PYCI_EUNSUPPORTED,
+               "pair count does not match the pair dimension");
+    PE_REQUIRE(P::lds <= cap, PYCI_EUNSUPPORTED, "the device has too little LDS for the matrix-free product");
+    DprodArgs A{(int)(nd * nvec), (int)D->nbs, nvec, D->npair, D->nea, ntot, (int)(D->na * D->nea),
+                (int)(D->na * D->nea + D->nbs * D->neb), P::ld};
+    if (ntot <= 64) launch_dprod_v<MT, T, P::SB, P::VM, 8>(D, A, X, V, kv, W, P::lds, s);
+    else if (ntot <= 96) launch_dprod_v<MT, T, P::SB, P::VM, 12>(D, A, X, V, kv, W, P::lds, s);
+    else launch_dprod_v<MT, T, P::SB, P::VM, 16>(D, A, X, V, kv, W, P::lds, s);
+}
+
+// ev (optional): 3 events recorded before, between and after the two passes.
 template <int NV, typename T>
-static void launch_passes(const DirectOp *D, i64 nd, int nvec, const T *X, T *W, i64 ldw, T *Y, const T *V,
-                          const T *kv, hipStream_t s, hipEvent_t *ev) {
-    const bool tiled = (D->nbs + 15) / 16 >= 16;  // enough column tiles to spread over the XCDs
-    const unsigned g = tiled ? tiled_grid(D->na, D->nbs) : grid_for(nd, 256);
+static void launch_passes(const DirectOp *D, i64 nd, int nvec, const T *X, T *W, T *Y, const T *V, const T *kv,
+                          hipStream_t s, hipEvent_t *ev) {
     if (ev) PE_HIP(hipEventRecord(ev[0], s));
-    if (tiled)
-        k_mf_d<NV, T, true><<<g, 256, 0, s>>>(nd, D->nbs, D->npair, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, X,
-                                              W, ldw);
-    else
-        k_mf_d<NV, T, false><<<g, 256, 0, s>>>(nd, D->nbs, D->npair, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, X,
-                                               W, ldw);
-    PE_LAUNCH_CHECK();
-    if (ev) PE_HIP(hipEventRecord(ev[1], s));
-    const i64 ncol = nd * nvec;
     switch (D->kpad / 16) {
-        case 1: launch_gemm<1, T>(ncol, V, kv, X, W, ldw, s); break;
-        case 2: launch_gemm<2, T>(ncol, V, kv, X, W, ldw, s); break;
-        case 3: launch_gemm<3, T>(ncol, V, kv, X, W, ldw, s); break;
-        case 4: launch_gemm<4, T>(ncol, V, kv, X, W, ldw, s); break;
-        case 5: launch_gemm<5, T>(ncol, V, kv, X, W, ldw, s); break;
-        case 6: launch_gemm<6, T>(ncol, V, kv, X, W, ldw, s); break;
-        case 7: launch_gemm<7, T>(ncol, V, kv, X, W, ldw, s); break;
-        case 8: launch_gemm<8, T>(ncol, V, kv, X, W, ldw, s); break;
+        case 1: launch_dprod<1, T>(D, nd, nvec, X, V, kv, W, s); break;
+        case 2: launch_dprod<2, T>(D, nd, nvec, X, V, kv, W, s); break;
+        case 3: launch_dprod<3, T>(D, nd, nvec, X, V, kv, W, s); break;
+        case 4: launch_dprod<4, T>(D, nd, nvec, X, V, kv, W, s); break;
+        case 5: launch_dprod<5, T>(D, nd, nvec, X, V, kv, W, s); break;
+        case 6: launch_dprod<6, T>(D, nd, nvec, X, V, kv, W, s); break;
+        case 7: launch_dprod<7, T>(D, nd, nvec, X, V, kv, W, s); break;
+        case 8: launch_dprod<8, T>(D, nd, nvec, X, V, kv, W, s); break;
         default: throw EngineError(PYCI_EUNSUPPORTED, "pair dimension too large for the matrix-free operator");
     }
-    if (ev) PE_HIP(hipEventRecord(ev[2], s));
+    if (ev) PE_HIP(hipEventRecord(ev[1], s));
+    const bool tiled = (D->nbs + 15) / 16 >= 16;  // enough column tiles to spread over the XCDs
+    const unsigned g = tiled ? tiled_grid(D->na, D->nbs) : grid_for(nd, 256);
     if (tiled)
-        k_mf_sigma<NV, T, true><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, W, ldw,
-                                                  Y);
+        k_mf_sigma<NV, T, true><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->kpad, D->tab_a.p, D->tab_b.p, nvec,
+                                                  W, Y);
     else
-        k_mf_sigma<NV, T, false><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->tab_a.p, D->tab_b.p, nvec, W, ldw,
-                                                   Y);
+        k_mf_sigma<NV, T, false><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->kpad, D->tab_a.p, D->tab_b.p,
+                                                   nvec, W, Y);
     PE_LAUNCH_CHECK();
-    if (ev) PE_HIP(hipEventRecord(ev[3], s));
+    if (ev) PE_HIP(hipEventRecord(ev[2], s));
 }
 
 template <typename T>
@@ -540,16 +676,14 @@ void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, 
     DirectOp *D = op->direct.get();
     PE_REQUIRE(nvec >= 1 && nvec <= 16, PYCI_EUNSUPPORTED, "the matrix-free operator takes 1..16 vectors");
     const i64 nd = op->nrow;
-    const i64 ldw = (nd * nvec + 127) / 128 * 128;
-    T *W = direct_workspace<T>(D, ldw, s);
+    T *W = direct_workspace<T>(D, (nd * nvec + 63) / 64 * 64);
     const T *V = std::is_same<T, double>::value ? (const T *)D->V64.p : (const T *)D->V32.p;
     const T *kv = std::is_same<T, double>::value ? (const T *)D->k64.p : (const T *)D->k32.p;
-    const i64 ldw_used = std::is_same<T, double>::value ? D->ldw64 : D->ldw32;
-    if (nvec == 1) launch_passes<1, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s, ev);
-    else if (nvec == 2) launch_passes<2, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s, ev);
-    else if (nvec <= 4) launch_passes<4, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s, ev);
-    else if (nvec <= 8) launch_passes<8, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s, ev);
-    else launch_passes<16, T>(D, nd, nvec, X, W, ldw_used, Y, V, kv, s, ev);
+    if (nvec == 1) launch_passes<1, T>(D, nd, nvec, X, W, Y, V, kv, s, ev);
+    else if (nvec == 2) launch_passes<2, T>(D, nd, nvec, X, W, Y, V, kv, s, ev);
+    else if (nvec <= 4) launch_passes<4, T>(D, nd, nvec, X, W, Y, V, kv, s, ev);
+    else if (nvec <= 8) launch_passes<8, T>(D, nd, nvec, X, W, Y, V, kv, s, ev);
+    else launch_passes<16, T>(D, nd, nvec, X, W, Y, V, kv, s, ev);
 }
 
 template void direct_sigma<double>(const pyci_op *, int, const double *, double *, hipStream_t, hipEvent_t *);
@@ -557,9 +691,8 @@ template void direct_sigma<float>(const pyci_op *, int, const float *, float *, 
 
 // Per-pass timing of nrep matrix-free sigmas on random X (see pyci_op_bench_sigma_parts).
 // Algorithmic bytes and flops per launch, with N = nd * nvec columns and T the value size:
-//   gather pass:  reads X (T N), writes D (T npair N)
-//   product:      reads D and X, writes E (T (2 npair + 1) N); 2 npair^2 N flops
-//   scatter pass: reads E (T npair N), writes Y (T N)
+//   gather + product: reads X (T N), writes E (T npair N); 2 npair^2 N flops
+//   scatter pass:     reads E (T npair N), writes Y (T N)
 void direct_bench_parts(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *ms, double *bytes, double *flops) {
     DirectOp *D = op->direct.get();
     hipStream_t s = op->ctx->stream;
@@ -574,7 +707,7 @@ void direct_bench_parts(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *ms, do
         k_direct_to_f32<<<1024, 256, 0, s>>>(x.p, x32.p, N);
         PE_LAUNCH_CHECK();
     }
-    std::vector<hipEvent_t> ev((size_t)(4 * nrep));
+    std::vector<hipEvent_t> ev((size_t)(3 * nrep));
     for (auto &e : ev) PE_HIP(hipEventCreate(&e));
     auto run = [&](hipEvent_t *e) {
         if (fp32) direct_sigma<float>(op, (int)nvec, x32.p, y32.p, s, e);
@@ -582,23 +715,21 @@ void direct_bench_parts(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *ms, do
     };
     try {
         run(nullptr);  // warm-up (workspace allocation)
-        for (i64 r = 0; r < nrep; ++r) run(&ev[(size_t)(4 * r)]);
+        for (i64 r = 0; r < nrep; ++r) run(&ev[(size_t)(3 * r)]);
         PE_HIP(hipEventSynchronize(ev.back()));
-        double acc[3] = {0, 0, 0};
+        double acc[2] = {0, 0};
         for (i64 r = 0; r < nrep; ++r)
-            for (int p = 0; p < 3; ++p) {
+            for (int p = 0; p < 2; ++p) {
                 float t = 0.f;
-                PE_HIP(hipEventElapsedTime(&t, ev[(size_t)(4 * r + p)], ev[(size_t)(4 * r + p + 1)]));
+                PE_HIP(hipEventElapsedTime(&t, ev[(size_t)(3 * r + p)], ev[(size_t)(3 * r + p + 1)]));
                 acc[p] += t;
             }
         const double T = fp32 ? 4.0 : 8.0, P = (double)D->npair, NN = (double)N;
-        for (int p = 0; p < 3; ++p) ms[p] = (float)(acc[p] / (double)nrep);
+        for (int p = 0; p < 2; ++p) ms[p] = (float)(acc[p] / (double)nrep);
         bytes[0] = T * NN * (1.0 + P);
-        bytes[1] = T * NN * (2.0 * P + 1.0);
-        bytes[2] = T * NN * (P + 1.0);
-        flops[0] = 2.0 * (double)(D->nea + D->neb) * NN;
-        flops[1] = 2.0 * P * P * NN;
-        flops[2] = 2.0 * (double)(D->nea + D->neb) * NN;
+        bytes[1] = T * NN * (P + 1.0);
+        flops[0] = 2.0 * P * P * NN + 2.0 * (double)(D->nea + D->neb) * NN;
+        flops[1] = 2.0 * (double)(D->nea + D->neb) * NN;
     } catch (...) {
         for (auto &e : ev) (void)hipEventDestroy(e);
         throw;

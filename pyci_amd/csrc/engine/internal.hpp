@@ -133,12 +133,13 @@ struct DirectOp {
     double ecore = 0.0;
     bool csr_ready = false;      // the CSR arrays of the operator have been materialised
     DevBuf<int2> tab_a, tab_b;   // [string][entry] {rank of the excited string, pair | negative << 16}
+    DevBuf<int2> tab;            // tab_a rows, tab_b rows, then the sentinel {0, -1}
     DevBuf<u64> str_a, str_b;    // strings in colex order
     DevBuf<double> one_mo, two_mo, V64, k64;
     DevBuf<float> V32, k32;
-    DevBuf<double> work64;       // D / E passes, kpad rows x ldw64 columns
+    DevBuf<double> work64;       // E values, tile-major: ceil(columns / 64) tiles of kpad x 64
     DevBuf<float> work32;
-    i64 ldw64 = 0, ldw32 = 0;
+    i64 ldw64 = 0, ldw32 = 0;    // columns the workspaces hold (multiple of 64)
 };
 }  // namespace pe
 

@@ -17,7 +17,9 @@ def fullci(nb, nu, nd, nchol=8, seed=3):
     return ham, wfn, one, two
 
 
-@pytest.mark.parametrize("nb,nu,nd", [(4, 1, 1), (6, 3, 3), (7, 3, 2), (8, 4, 4), (9, 5, 2), (11, 5, 5)])
+# nbasis 13..15: pair dimensions 96..128 (the single-buffer and cached-V plans of the fused pass)
+@pytest.mark.parametrize("nb,nu,nd", [(4, 1, 1), (6, 3, 3), (7, 3, 2), (8, 4, 4), (9, 5, 2), (11, 5, 5), (13, 2, 2),
+                                      (14, 2, 1), (15, 2, 2)])
 def test_direct_matvec_matches_csr(nb, nu, nd):
     ham, wfn, _, _ = fullci(nb, nu, nd, seed=nb)
     a = pyci.sparse_op(ham, wfn, storage="csr")
@@ -96,3 +98,15 @@ def test_direct_be_fullci_energy():
     assert op.storage == "direct"
     es, _ = op.solve(n=1, tol=1e-9)
     assert abs(es[0] - (-14.617409507)) < 1e-8
+
+
+@pytest.mark.parametrize("nb,nu,nd", [(12, 2, 2), (13, 2, 2), (14, 2, 1), (15, 2, 1)])
+def test_direct_mixed_solve_large_pair_dimension(nb, nu, nd):
+    """FP32 sigma of the fused pass at pair dimensions 80..128, through the mixed
+    solve, against the FP64 CSR solve."""
+    ham, wfn, _, _ = fullci(nb, nu, nd, seed=nb + 40)
+    ref, _ = pyci.sparse_op(ham, wfn, storage="csr").solve(n=2, tol=1e-10)
+    op = pyci.sparse_op(ham, wfn)
+    assert op.storage == "direct"
+    es, cs = op.solve(n=2, tol=1e-10, precision="mixed")
+    np.testing.assert_allclose(es, ref, rtol=0, atol=1e-9)
