@@ -25,9 +25,50 @@ namespace pe {
 
 constexpr int DT = 256;
 
-// partial[b][ma][mb] = sum over rows of block b of A[:, i] * B[:, j] (8x8 tile per blockIdx.y)
+// Sums over the per-block partials of an earlier kernel (partial[b * cnt + t],
+// b < npart), computed by every block of a consuming kernel into shared memory
+// out[t] (t < cnt): a fixed order, so all blocks (and launches) agree bit for bit.
+__device__ void reduce_partials_block(const double *partial, int npart, int cnt, double *out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int t = wave; t < cnt; t += nw) {
+        double v = 0.0;
+        for (int b = lane; b < npart; b += 64) v += partial[(i64)b * cnt + t];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) out[t] = v;
+    }
+    __syncthreads();
+}
+
+// Sum of v over the block, written by thread 0 to *dst.
+__device__ inline void block_sum_to(double v, double *dst) {
+    __shared__ double red[DT / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < DT / 64; ++w) t += red[w];
+        *dst = t;
+    }
+}
+
+// Second classical Gram-Schmidt pass only where the first removed more than half
+// of the squared norm (|t1|^2 < |t0|^2 / 2): "twice is enough" with the
+// Daniel-Gragg-Kaufman-Stewart test, decided on the device from the partial norms
+// of the copy (p0) and of the first pass (p1).
+__device__ bool second_pass(const double *p0, const double *p1, int npart) {
+    __shared__ double nn[2];
+    reduce_partials_block(p0, npart, 1, nn);
+    reduce_partials_block(p1, npart, 1, nn + 1);
+    return nn[1] < 0.5 * nn[0];
+}
+
+// partial[b][ma][mb] = sum over rows of block b of A[:, i] * B[:, j] (8x8 tile per
+// blockIdx.y). With gate0 set the launch is the second Gram-Schmidt pass and does
+// nothing when the first pass sufficed.
 __global__ void __launch_bounds__(DT) k_gram(i64 nd, const double *A, i64 lda, int ma, const double *B, i64 ldb,
-                                             int mb, double *partial) {
+                                             int mb, double *partial, const double *gate0, const double *gate1) {
+    if (gate0 && !second_pass(gate0, gate1, gridDim.x)) return;
     const int tiles_j = (mb + 7) / 8;
     const int i0 = (blockIdx.y / tiles_j) * 8, j0 = (blockIdx.y % tiles_j) * 8;
     const int ni = min(8, ma - i0), nj = min(8, mb - j0);
@@ -90,10 +131,59 @@ __global__ void k_gemm_update(i64 nd, const double *V, i64 ldv, int m, const dou
     }
 }
 
-// Ritz vectors, images, preconditioned residuals and residual norm partials.
+// T[:, j] = V[:, 0:m] C[:, j]
+__global__ void k_gemm_set(i64 nd, const double *V, i64 ldv, int m, const double *C, int k, double *T, i64 ldt) {
+    for (i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x; r < nd; r += (i64)gridDim.x * blockDim.x) {
+        for (int j = 0; j < k; ++j) {
+            double acc = 0.0;
+            for (int i = 0; i < m; ++i) acc += V[(i64)i * ldv + r] * C[j * m + i];
+            T[(i64)j * ldt + r] = acc;
+        }
+    }
+}
+
+// t -= V[:, 0:m] c with c the sums of cpart (partials of k_gram, m per block),
+// and the partial squared norms of the result per block; a gated launch (second
+// Gram-Schmidt pass) does nothing when the first pass sufficed. m <= 128.
+__global__ void __launch_bounds__(DT) k_update_norm(i64 nd, const double *V, int m, const double *cpart, double *t,
+                                                    double *npart, const double *gate0, const double *gate1) {
+    if (gate0 && !second_pass(gate0, gate1, gridDim.x)) return;
+    __shared__ double c[128];
+    reduce_partials_block(cpart, gridDim.x, m, c);
+    double ss = 0.0;
+    for (i64 r = (i64)blockIdx.x * DT + threadIdx.x; r < nd; r += (i64)gridDim.x * DT) {
+        double acc = 0.0;
+        for (int i = 0; i < m; ++i) acc += V[(i64)i * nd + r] * c[i];
+        const double x = t[r] - acc;
+        t[r] = x;
+        ss += x * x;
+    }
+    block_sum_to(ss, npart + blockIdx.x);
+}
+
+// dst = src, or dst = (src + sgn P src) / 2 with P the alpha <-> beta exchange of
+// a full FullCI space in colex order with nup == ndn (index Ja * nb + Jb <->
+// Jb * nb + Ja) when sgn != 0; the partial squared norms of dst per block.
+__global__ void __launch_bounds__(DT) k_put(const double *src, double *dst, i64 nd, i64 nb, double sgn,
+                                            double *npart) {
+    double ss = 0.0;
+    for (i64 J = (i64)blockIdx.x * DT + threadIdx.x; J < nd; J += (i64)gridDim.x * DT) {
+        double x = src[J];
+        if (sgn != 0.0) {
+            const i64 a = J / nb, b = J - a * nb;
+            x = 0.5 * (x + sgn * src[b * nb + a]);
+        }
+        dst[J] = x;
+        ss += x * x;
+    }
+    block_sum_to(ss, npart + blockIdx.x);
+}
+
+// Ritz vectors and images (only when write_xw), preconditioned residuals and the
+// squared residual norms (completed in-kernel into res).
 __global__ void __launch_bounds__(DT) k_ritz(i64 nd, const double *V, const double *W, i64 ld, int m,
                                              const double *Y, const double *theta, int k, const double *diag,
-                                             double *X, double *WX, double *T, double *npart) {
+                                             int write_xw, double *X, double *WX, double *T, double *npart) {
     double ss[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) ss[j] = 0.0;
@@ -118,8 +208,10 @@ __global__ void __launch_bounds__(DT) k_ritz(i64 nd, const double *V, const doub
                 const double res = w[j] - theta[j] * x[j];
                 double den = theta[j] - d;
                 if (fabs(den) < 1e-8) den = (den < 0.0) ? -1e-8 : 1e-8;
-                X[(i64)j * ld + r] = x[j];
-                WX[(i64)j * ld + r] = w[j];
+                if (write_xw) {
+                    X[(i64)j * ld + r] = x[j];
+                    WX[(i64)j * ld + r] = w[j];
+                }
                 T[(i64)j * ld + r] = res / den;
                 ss[j] += res * res;
             }
@@ -147,18 +239,36 @@ __global__ void k_scale(double *x, i64 n, double s) {
 // Normalises x by the norm whose square is nn[1]; a vector whose norm fell below
 // 1e-10 of its norm before orthogonalisation (square in nn[0]) is numerically
 // dependent on the basis and is zeroed instead (the host drops it later).
-__device__ inline double checked_scale(const double *nn) {
-    const double n0 = sqrt(fmax(nn[0], 0.0)), n1 = sqrt(fmax(nn[1], 0.0));
+__device__ inline double checked_scale(double n0sq, double n1sq) {
+    const double n0 = sqrt(fmax(n0sq, 0.0)), n1 = sqrt(fmax(n1sq, 0.0));
     return (n1 > 1e-10 * fmax(n0, 1e-300) && n1 > 1e-300) ? 1.0 / n1 : 0.0;
 }
 
 __global__ void k_scale_checked(double *x, i64 n, const double *nn) {
-    const double s = checked_scale(nn);
+    const double s = checked_scale(nn[0], nn[1]);
     for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) x[i] *= s;
 }
 
-// dst = (src + s P src) / 2 with P the alpha <-> beta exchange of a full FullCI
-// space in colex order with nup == ndn: index Ja * nb + Jb <-> Jb * nb + Ja.
+// Final normalisation of an orthogonalised correction from the partial squared
+// norms (npart per block) before Gram-Schmidt (p0), after the first pass (p1) and
+// after the second (p2, used when `gated` and the second pass ran). Writes the
+// pair (before, final) to pair for the host.
+__global__ void __launch_bounds__(DT) k_scale_final(double *x, i64 n, const double *p0, const double *p1,
+                                                    const double *p2, int npart, int gated, double *pair) {
+    __shared__ double nn[3];
+    reduce_partials_block(p0, npart, 1, nn);
+    reduce_partials_block(p1, npart, 1, nn + 1);
+    if (gated) reduce_partials_block(p2, npart, 1, nn + 2);
+    const double nf = (gated && nn[1] < 0.5 * nn[0]) ? nn[2] : nn[1];
+    const double s = checked_scale(nn[0], nf);
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) x[i] *= s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        pair[0] = nn[0];
+        pair[1] = nf;
+    }
+}
+
+// dst = (src + s P src) / 2 (see k_put), without the norm.
 __global__ void k_parity(const double *src, double *dst, i64 nd, i64 nb, double sgn) {
     for (i64 J = (i64)blockIdx.x * blockDim.x + threadIdx.x; J < nd; J += (i64)gridDim.x * blockDim.x) {
         const i64 a = J / nb, b = J - a * nb;
@@ -239,7 +349,11 @@ struct Davidson {
     i64 nd;
     int k, mmax;
     DevBuf<double> V, W, X, WX, T, rowbuf, rowout, partial, small;
-    DevBuf<double> RV, RW, dnorm;  // thick-restart scratch (nd x 2k each); per-vector norm pairs
+    DevBuf<double> RV, RW;   // thick-restart scratch (nd x 2k each)
+    DevBuf<double> dv;       // small device operands and results (layout: the o* offsets)
+    HostBuf<double> hb;      // page-locked mirror of dv for the per-iteration transfers
+    DevBuf<double> pb;       // per-block partials of the residual norms, the Gram-Schmidt
+                             // coefficients and the correction norms (layout: the p* offsets)
     DevBuf<float> row32, out32;
     std::vector<double> G;  // mmax x mmax row-major
     int m = 0;
@@ -247,6 +361,12 @@ struct Davidson {
     int sector = 0;   // 0, or +1 / -1: the basis is kept in that parity sector of P
     i64 swap_nb = 0;  // beta-string count of P's index map
     pyci_solve_stats st{};
+    // dv / hb layout (doubles): Ritz coefficients Y (m x k) and theta go down; the
+    // squared residual norms, the new G block and the norm pairs of the added
+    // columns come back in one copy
+    size_t oY, oTH, oYK, oRES, oC1, oC2, oNRM, oPAIR, oG, osz;
+    size_t pRES, pN0, pN1, pN2, pC1, pC2;  // in pb: nblk entries per block-partial quantity
+    static constexpr int kCgsMax = 128;    // basis size up to which Gram-Schmidt runs on partials
 
     Davidson(pyci_op *o, int k_, int mmax_) : op(o), s(o->ctx->stream), nd(o->nrow), k(k_), mmax(mmax_) {
         V.alloc((size_t)(nd * mmax));
@@ -259,42 +379,42 @@ struct Davidson {
         rowbuf.alloc((size_t)(nd * kcols));
         rowout.alloc((size_t)(nd * kcols));
         nblk = (int)std::min<i64>(512, (nd + DT - 1) / DT);
-        partial.alloc((size_t)nblk * (size_t)std::max(mmax * k, 64) + 64);
+        partial.alloc((size_t)nblk * (size_t)std::max(mmax * std::max(k, kcols), 64) + 64);
         small.alloc((size_t)(mmax * mmax + 4 * mmax + 64));
-        dnorm.alloc(64);
+        const size_t M = (size_t)mmax;
+        oY = 0;
+        oTH = oY + M * 16;
+        oYK = oTH + 16;
+        oRES = oYK + M * 32;
+        oC1 = oRES + 16;
+        oC2 = oC1 + M;
+        oNRM = oC2 + M;
+        oPAIR = oNRM + 3 * 16;
+        oG = oPAIR + 2 * 16;
+        osz = oG + M * 16;
+        dv.alloc(osz);
+        hb.alloc(osz);
+        const size_t NB = (size_t)nblk;
+        pRES = 0;
+        pN0 = pRES + NB * 16;
+        pN1 = pN0 + NB;
+        pN2 = pN1 + NB;
+        pC1 = pN2 + NB;
+        pC2 = pC1 + NB * kCgsMax;
+        pb.alloc(pC2 + NB * kCgsMax);
         G.assign((size_t)mmax * mmax, 0.0);
     }
 
     unsigned g1() const { return (unsigned)std::min<i64>(4096, (nd + 255) / 256); }
 
-    // out (ma x mb, column-major on the device at `dst`) = A^T B
+    // out (ma x mb, row-major [i][j] on the device) = A^T B
     void gram(const double *A, int ma, const double *B, int mb, double *dst) {
         dim3 grid((unsigned)nblk, (unsigned)(((ma + 7) / 8) * ((mb + 7) / 8)));
-        k_gram<<<grid, DT, 0, s>>>(nd, A, nd, ma, B, nd, mb, partial.p);
+        k_gram<<<grid, DT, 0, s>>>(nd, A, nd, ma, B, nd, mb, partial.p, nullptr, nullptr);
         PE_LAUNCH_CHECK();
-        int cnt = ma * mb;
+        const int cnt = ma * mb;
         k_reduce_partials<<<(unsigned)cnt, 256, 0, s>>>(partial.p, nblk, cnt, dst);
         PE_LAUNCH_CHECK();
-        // dst is [i][j] row-major (i < ma); as column-major m x k with m = ma it is
-        // transposed, so callers that need C (ma x mb col-major) use gram_cm.
-    }
-
-    // C (ma x mb column-major, on device) = A^T B
-    void gram_cm(const double *A, int ma, const double *B, int mb, double *C) {
-        if (mb == 1) {
-            // [ma][1] row-major is already ma x 1 column-major: no host round trip
-            gram(A, ma, B, mb, C);
-            return;
-        }
-        gram(A, ma, B, mb, small.p + (size_t)mmax * mmax);
-        // transpose row-major [ma][mb] to column-major via host (tiny)
-        std::vector<double> h((size_t)ma * mb), t((size_t)ma * mb);
-        PE_HIP(hipMemcpyAsync(h.data(), small.p + (size_t)mmax * mmax, sizeof(double) * ma * mb,
-                              hipMemcpyDeviceToHost, s));
-        PE_HIP(hipStreamSynchronize(s));
-        for (int i = 0; i < ma; ++i)
-            for (int j = 0; j < mb; ++j) t[(size_t)j * ma + i] = h[(size_t)i * mb + j];
-        PE_HIP(hipMemcpyAsync(C, t.data(), sizeof(double) * ma * mb, hipMemcpyHostToDevice, s));
     }
 
     std::vector<double> gram_host(const double *A, int ma, const double *B, int mb) {
@@ -332,13 +452,14 @@ struct Davidson {
         st.sigma_calls += cnt;
     }
 
-    // Orthogonalise column V[:, m] against V[:, 0:m] twice and normalise.
-    // Returns false (column rejected) if it is numerically dependent.
+    // Orthogonalise column V[:, m] against V[:, 0:m] twice and normalise (start
+    // vectors; host round trips). Returns false (column rejected) if it is
+    // numerically dependent.
     bool orthonormalize_new(double norm0) {
         double *col = V.p + (i64)m * nd;
         for (int pass = 0; pass < 2 && m > 0; ++pass) {
-            double *C = small.p + (size_t)mmax * mmax + 2 * mmax;  // m coefficients
-            gram_cm(V.p, m, col, 1, C);
+            double *C = dv.p + oC1;
+            gram(V.p, m, col, 1, C);
             k_gemm_update<<<g1(), 256, 0, s>>>(nd, V.p, nd, m, C, 1, -1.0, col, nd);
             PE_LAUNCH_CHECK();
         }
@@ -350,33 +471,64 @@ struct Davidson {
         return true;
     }
 
-    // Appends column V[:, m] (already copied in) orthogonalised twice against
-    // V[:, 0:m] and normalised, with no host round trip: the norms before and
-    // after go to dnorm[2 slot], dnorm[2 slot + 1]; a dependent vector is zeroed.
+    // V[:, m] = correction j (kept in the basis' parity sector), the partials of its
+    // squared norm to pN0
+    void put_correction(int j) {
+        k_put<<<(unsigned)nblk, DT, 0, s>>>(T.p + (i64)j * nd, V.p + (i64)m * nd, nd, swap_nb,
+                                            sector != 0 ? (double)sector : 0.0, pb.p + pN0);
+        PE_LAUNCH_CHECK();
+    }
+
+    // Appends column V[:, m] (filled by put_correction) orthogonalised against
+    // V[:, 0:m] (classical Gram-Schmidt, the second pass only where the first lost
+    // more than half the squared norm) and normalised, all on the device; the norm
+    // pair (before, after) goes to the slot's pair for the host's dependency check
+    // and a dependent vector is zeroed.
     void orthonormalize_device(int slot) {
         double *col = V.p + (i64)m * nd;
-        gram(col, 1, col, 1, dnorm.p + 2 * slot);
-        for (int pass = 0; pass < 2 && m > 0; ++pass) {
-            double *C = small.p + (size_t)mmax * mmax + 2 * mmax;
-            gram_cm(V.p, m, col, 1, C);
-            k_gemm_update<<<g1(), 256, 0, s>>>(nd, V.p, nd, m, C, 1, -1.0, col, nd);
+        double *pair = dv.p + oPAIR + 2 * slot;
+        const double *p0 = pb.p + pN0;
+        double *p1 = pb.p + pN1, *p2 = pb.p + pN2;
+        if (m == 0) {
+            k_scale_final<<<g1(), DT, 0, s>>>(col, nd, p0, p0, p0, nblk, 0, pair);
+        } else if (m <= kCgsMax) {
+            const dim3 grid((unsigned)nblk, (unsigned)((m + 7) / 8));
+            k_gram<<<grid, DT, 0, s>>>(nd, V.p, nd, m, col, nd, 1, pb.p + pC1, nullptr, nullptr);
             PE_LAUNCH_CHECK();
+            k_update_norm<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, m, pb.p + pC1, col, p1, nullptr, nullptr);
+            PE_LAUNCH_CHECK();
+            k_gram<<<grid, DT, 0, s>>>(nd, V.p, nd, m, col, nd, 1, pb.p + pC2, p0, p1);
+            PE_LAUNCH_CHECK();
+            k_update_norm<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, m, pb.p + pC2, col, p2, p0, p1);
+            PE_LAUNCH_CHECK();
+            k_scale_final<<<g1(), DT, 0, s>>>(col, nd, p0, p1, p2, nblk, 1, pair);
+        } else {
+            // large bases: both passes unconditionally, coefficients reduced apart
+            for (int pass = 0; pass < 2; ++pass) {
+                gram(V.p, m, col, 1, dv.p + oC1);
+                k_gemm_update<<<g1(), 256, 0, s>>>(nd, V.p, nd, m, dv.p + oC1, 1, -1.0, col, nd);
+                PE_LAUNCH_CHECK();
+            }
+            k_put<<<(unsigned)nblk, DT, 0, s>>>(col, col, nd, swap_nb, 0.0, p1);
+            PE_LAUNCH_CHECK();
+            k_scale_final<<<g1(), DT, 0, s>>>(col, nd, p0, p1, p1, nblk, 0, pair);
         }
-        gram(col, 1, col, 1, dnorm.p + 2 * slot + 1);
-        k_scale_checked<<<g1(), 256, 0, s>>>(col, nd, dnorm.p + 2 * slot);
         PE_LAUNCH_CHECK();
         ++m;
     }
 
-    // Fills G for the cnt new columns m0..m0+cnt-1 (one device-to-host copy that
-    // also brings the norm pairs of orthonormalize_device) and removes the columns
-    // that were zeroed as dependent. Returns the number of columns kept.
-    int update_G_checked(int m0, int cnt) {
-        gram(V.p, m0 + cnt, W.p + (i64)m0 * nd, cnt, small.p);
-        std::vector<double> h((size_t)(m0 + cnt) * cnt), nn((size_t)(2 * cnt));
-        PE_HIP(hipMemcpyAsync(h.data(), small.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
-        PE_HIP(hipMemcpyAsync(nn.data(), dnorm.p, sizeof(double) * nn.size(), hipMemcpyDeviceToHost, s));
+    // Copies the results of this iteration (residual norms; with cnt new columns
+    // their G block and norm pairs) to the host and waits for them.
+    void readback(int cnt) {
+        const size_t last = cnt > 0 ? oG + (size_t)m * cnt : oRES + (size_t)k;
+        PE_HIP(hipMemcpyAsync(hb.p + oRES, dv.p + oRES, sizeof(double) * (last - oRES), hipMemcpyDeviceToHost, s));
         PE_HIP(hipStreamSynchronize(s));
+    }
+
+    // Fills G for the cnt new columns m0..m0+cnt-1 from the read-back block and
+    // removes the columns that were zeroed as dependent. Returns the number kept.
+    int accept_G(int m0, int cnt) {
+        const double *h = hb.p + oG, *nn = hb.p + oPAIR;
         std::vector<int> keep;
         for (int j = 0; j < cnt; ++j) {
             const double n0 = std::sqrt(std::max(nn[2 * j], 0.0)), n1 = std::sqrt(std::max(nn[2 * j + 1], 0.0));
@@ -430,38 +582,104 @@ struct Davidson {
             }
     }
 
+    // Residuals of the Ritz pairs (y: m x m eigenvectors of G, row-major; the first
+    // k columns are used), with the Ritz vectors and images written when write_xw.
+    void ritz(const std::vector<double> &y, const std::vector<double> &theta, bool write_xw) {
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < m; ++i) hb.p[oY + (size_t)j * m + i] = y[(size_t)i * m + j];
+        for (int j = 0; j < k; ++j) hb.p[oTH + j] = theta[(size_t)j];
+        PE_HIP(hipMemcpyAsync(dv.p + oY, hb.p + oY, sizeof(double) * (oTH + 16 - oY), hipMemcpyHostToDevice, s));
+        k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, m, dv.p + oY, dv.p + oTH, k, op->diag.p,
+                                            write_xw ? 1 : 0, X.p, WX.p, T.p, pb.p + pRES);
+        PE_LAUNCH_CHECK();
+        k_reduce_partials<<<(unsigned)k, 256, 0, s>>>(pb.p + pRES, nblk, k, dv.p + oRES);
+        PE_LAUNCH_CHECK();
+    }
+
+    // Ritz vectors X, WX of the pairs whose residuals were just read: recomputed
+    // from the coefficients still on the device, or, after a restart in the same
+    // iteration, the leading columns of the restarted basis.
+    void finish_x(bool restarted) {
+        if (restarted) {
+            PE_HIP(hipMemcpyAsync(X.p, V.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
+            PE_HIP(hipMemcpyAsync(WX.p, W.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
+            return;
+        }
+        k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, mx, dv.p + oY, dv.p + oTH, k, op->diag.p, 1, X.p,
+                                            WX.p, T.p, pb.p + pRES);
+        PE_LAUNCH_CHECK();
+    }
+    int mx = 0;  // basis size of the coefficients on the device (for finish_x)
+
+    // Thick restart on the lowest kk Ritz vectors: they are orthonormal and G
+    // becomes diag(w).
+    void restart(int kk, const std::vector<double> &w, const std::vector<double> &y) {
+        if (kk > k) {
+            if (!RV.p) {
+                RV.alloc((size_t)(nd * 2 * k));
+                RW.alloc((size_t)(nd * 2 * k));
+            }
+            for (int j = 0; j < kk; ++j)
+                for (int i = 0; i < m; ++i) hb.p[oYK + (size_t)j * m + i] = y[(size_t)i * m + j];
+            PE_HIP(hipMemcpyAsync(dv.p + oYK, hb.p + oYK, sizeof(double) * (size_t)m * kk, hipMemcpyHostToDevice, s));
+            k_gemm_set<<<g1(), 256, 0, s>>>(nd, V.p, nd, m, dv.p + oYK, kk, RV.p, nd);
+            PE_LAUNCH_CHECK();
+            k_gemm_set<<<g1(), 256, 0, s>>>(nd, W.p, nd, m, dv.p + oYK, kk, RW.p, nd);
+            PE_LAUNCH_CHECK();
+            PE_HIP(hipMemcpyAsync(V.p, RV.p, sizeof(double) * nd * kk, hipMemcpyDeviceToDevice, s));
+            PE_HIP(hipMemcpyAsync(W.p, RW.p, sizeof(double) * nd * kk, hipMemcpyDeviceToDevice, s));
+        } else {
+            // the Ritz vectors of this iteration are the whole restart
+            k_gemm_set<<<g1(), 256, 0, s>>>(nd, V.p, nd, m, dv.p + oY, k, X.p, nd);
+            PE_LAUNCH_CHECK();
+            k_gemm_set<<<g1(), 256, 0, s>>>(nd, W.p, nd, m, dv.p + oY, k, WX.p, nd);
+            PE_LAUNCH_CHECK();
+            PE_HIP(hipMemcpyAsync(V.p, X.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
+            PE_HIP(hipMemcpyAsync(W.p, WX.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
+        }
+        std::fill(G.begin(), G.end(), 0.0);
+        for (int j = 0; j < kk; ++j) G[(size_t)j * mmax + j] = w[(size_t)j];
+        m = kk;
+    }
+
     // Runs Davidson until every root meets tol (or maxiter iterations). With
     // fp32 set it stops early once residuals reach the FP32 floor (switch_tol).
-    // Returns true when converged to tol.
+    // Returns true when converged to tol; X, WX then hold the Ritz pairs.
+    // One root: the correction of an iteration is orthogonalised, multiplied and
+    // projected before its residual is known, so that an iteration needs a single
+    // host round trip (the last iteration's correction is discarded).
     bool run(double tol, i64 maxiter, bool fp32, double switch_tol, std::vector<double> &theta, i64 &iters) {
         const double eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
         std::vector<double> best_res(k, 1e300);
         int stall = 0;
+        const bool spec = (k == 1);
         for (;;) {
             std::vector<double> gm((size_t)m * m), w, y;
             for (int i = 0; i < m; ++i)
                 for (int j = 0; j < m; ++j) gm[(size_t)i * m + j] = G[(size_t)i * mmax + j];
             jacobi_eig(m, gm, w, y);
             theta.assign(w.begin(), w.begin() + k);
-            // Y (m x k col-major) and theta to the device
-            std::vector<double> yk((size_t)m * k);
-            for (int j = 0; j < k; ++j)
-                for (int i = 0; i < m; ++i) yk[(size_t)j * m + i] = y[(size_t)i * m + j];
-            double *dY = small.p, *dth = small.p + (size_t)m * k;
-            PE_HIP(hipMemcpyAsync(dY, yk.data(), sizeof(double) * m * k, hipMemcpyHostToDevice, s));
-            PE_HIP(hipMemcpyAsync(dth, theta.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-            k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, m, dY, dth, k, op->diag.p, X.p, WX.p, T.p,
-                                                partial.p);
-            PE_LAUNCH_CHECK();
-            std::vector<double> np((size_t)nblk * k);
-            PE_HIP(hipMemcpyAsync(np.data(), partial.p, sizeof(double) * nblk * k, hipMemcpyDeviceToHost, s));
-            PE_HIP(hipStreamSynchronize(s));
+            ritz(y, theta, false);
+            mx = m;
+            bool restarted = false;
+            int m0 = m, nspec = 0;
+            if (spec && m < nd) {
+                if (m + 1 > mmax) {
+                    restart(std::max(k, std::min({m, 2 * k, mmax - 1})), w, y);
+                    restarted = true;
+                }
+                m0 = m;
+                put_correction(0);
+                orthonormalize_device(0);
+                apply(m0, 1, fp32);
+                gram(V.p, m0 + 1, W.p + (i64)m0 * nd, 1, dv.p + oG);
+                nspec = 1;
+            }
+            readback(nspec);
             std::vector<double> res(k, 0.0);
-            for (int b = 0; b < nblk; ++b)
-                for (int j = 0; j < k; ++j) res[j] += np[(size_t)b * k + j];
             bool all = true, floor = true;
             for (int j = 0; j < k; ++j) {
-                res[j] = std::sqrt(res[j]);
+                res[j] = std::sqrt(std::max(hb.p[oRES + j], 0.0));
                 if (j < 64) st.residual[j] = res[j];
                 const double lim = tol * std::max(std::fabs(theta[j]), eps23);
                 if (!(res[j] <= lim)) all = false;
@@ -469,11 +687,16 @@ struct Davidson {
             }
             ++iters;
             ++st.iterations;
-            if (all) return true;
-            if (fp32 && floor) return false;
-            if (m >= nd) {
+            auto leave = [&](bool ok) {
+                m = m0;  // a speculative column is dropped
+                finish_x(restarted);
+                return ok;
+            };
+            if (all) return leave(true);
+            if (fp32 && floor) return leave(false);
+            if (m0 >= nd) {
                 // the basis spans the whole space: the Ritz pairs are exact to rounding
-                return true;
+                return leave(true);
             }
             // stagnation guard: residuals at the rounding floor stop improving
             bool improved = false;
@@ -487,73 +710,48 @@ struct Davidson {
             // residual at the rounding floor is accepted after 25, and only 300 flat
             // iterations count as failure (slow roots can plateau for dozens of
             // iterations before they converge)
-            if (fp32 && stall >= 8) return false;
+            if (fp32 && stall >= 8) return leave(false);
             if (!fp32 && stall >= 25) {
                 bool tiny = true;
                 for (int j = 0; j < k; ++j)
                     if (!(res[j] <= 1e-7 * std::max(std::fabs(theta[j]), 1.0))) tiny = false;
-                if (tiny) return true;
-                if (stall >= 300) return false;
+                if (tiny) return leave(true);
+                if (stall >= 300) return leave(false);
             }
-            if (iters >= maxiter) return false;
-            // unconverged roots contribute corrections
-            std::vector<int> add;
-            for (int j = 0; j < k; ++j)
-                if (!(res[j] <= tol * std::max(std::fabs(theta[j]), eps23))) add.push_back(j);
-            if (m + (int)add.size() > mmax) {
-                // thick restart on the lowest kk Ritz vectors (kk = 2k when room is left
-                // for the next corrections): they are orthonormal and G becomes diag(w)
-                const int kk = std::max(k, std::min({m, 2 * k, mmax - (int)add.size()}));
-                if (kk > k) {
-                    if (!RV.p) {
-                        RV.alloc((size_t)(nd * 2 * k));
-                        RW.alloc((size_t)(nd * 2 * k));
-                    }
-                    std::vector<double> ykk((size_t)m * kk);
-                    for (int j = 0; j < kk; ++j)
-                        for (int i = 0; i < m; ++i) ykk[(size_t)j * m + i] = y[(size_t)i * m + j];
-                    double *dYk = small.p;
-                    PE_HIP(hipMemcpyAsync(dYk, ykk.data(), sizeof(double) * ykk.size(), hipMemcpyHostToDevice, s));
-                    PE_HIP(hipMemsetAsync(RV.p, 0, sizeof(double) * nd * kk, s));
-                    PE_HIP(hipMemsetAsync(RW.p, 0, sizeof(double) * nd * kk, s));
-                    k_gemm_update<<<g1(), 256, 0, s>>>(nd, V.p, nd, m, dYk, kk, 1.0, RV.p, nd);
-                    PE_LAUNCH_CHECK();
-                    k_gemm_update<<<g1(), 256, 0, s>>>(nd, W.p, nd, m, dYk, kk, 1.0, RW.p, nd);
-                    PE_LAUNCH_CHECK();
-                    PE_HIP(hipMemcpyAsync(V.p, RV.p, sizeof(double) * nd * kk, hipMemcpyDeviceToDevice, s));
-                    PE_HIP(hipMemcpyAsync(W.p, RW.p, sizeof(double) * nd * kk, hipMemcpyDeviceToDevice, s));
-                    PE_HIP(hipStreamSynchronize(s));  // ykk (host) must outlive the copy
-                } else {
-                    PE_HIP(hipMemcpyAsync(V.p, X.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
-                    PE_HIP(hipMemcpyAsync(W.p, WX.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
+            if (iters >= maxiter) return leave(false);
+            if (spec) {
+                if (nspec) accept_G(m0, 1);
+            } else {
+                // unconverged roots contribute corrections
+                std::vector<int> add;
+                for (int j = 0; j < k; ++j)
+                    if (!(res[j] <= tol * std::max(std::fabs(theta[j]), eps23))) add.push_back(j);
+                if (m + (int)add.size() > mmax) {
+                    restart(std::max(k, std::min({m, 2 * k, mmax - (int)add.size()})), w, y);
+                    restarted = true;
                 }
-                std::fill(G.begin(), G.end(), 0.0);
-                for (int j = 0; j < kk; ++j) G[(size_t)j * mmax + j] = w[(size_t)j];
-                m = kk;
-            }
-            const int m0 = m;
-            int slot = 0;
-            for (int j : add) {
-                if (m >= mmax || m >= nd) break;
-                if (sector != 0) {
-                    // keep the basis in its sector (rounding would otherwise let it drift)
-                    k_parity<<<g1(), 256, 0, s>>>(T.p + (i64)j * nd, V.p + (i64)m * nd, nd, swap_nb, (double)sector);
-                    PE_LAUNCH_CHECK();
-                } else {
-                    PE_HIP(hipMemcpyAsync(V.p + (i64)m * nd, T.p + (i64)j * nd, sizeof(double) * nd,
-                                          hipMemcpyDeviceToDevice, s));
+                m0 = m;
+                int slot = 0;
+                for (int j : add) {
+                    if (m >= mmax || m >= nd) break;
+                    put_correction(j);
+                    orthonormalize_device(slot++);
                 }
-                orthonormalize_device(slot++);
-            }
-            if (m > m0) {
-                apply(m0, m - m0, fp32);
-                update_G_checked(m0, m - m0);
+                if (m > m0) {
+                    const int cnt = m - m0;
+                    apply(m0, cnt, fp32);
+                    gram(V.p, m, W.p + (i64)m0 * nd, cnt, dv.p + oG);
+                    readback(cnt);
+                    accept_G(m0, cnt);
+                }
             }
             if (m == m0) {
                 // no usable direction: accept if the residuals are tiny, else fail
                 bool tiny = true;
                 for (int j = 0; j < k; ++j)
                     if (!(res[j] <= 1e-7 * std::max(std::fabs(theta[j]), 1.0))) tiny = false;
+                // the Ritz pairs of this iteration (the coefficients are still on the device)
+                finish_x(restarted);
                 return tiny && !fp32;
             }
         }

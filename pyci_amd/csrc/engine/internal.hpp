@@ -90,6 +90,35 @@ struct DevBuf {
     }
 };
 
+// Page-locked host buffer (hipHostMalloc): copies to and from it run
+// asynchronously on the stream instead of staging through a pageable bounce.
+template <typename T>
+struct HostBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    ~HostBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw EngineError(e == hipErrorOutOfMemory ? PYCI_ENOMEM : PYCI_EHIP,
+                              "hipHostMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " +
+                                  hipGetErrorString(e));
+        }
+        n = count;
+    }
+    void release() {
+        if (p && hipHostFree(p) != hipSuccess) (void)hipGetLastError();
+        p = nullptr;
+        n = 0;
+    }
+};
+
 }  // namespace pe
 
 // ---------------------------------------------------------------------------
