@@ -131,6 +131,17 @@ __global__ void k_gemm_update(i64 nd, const double *V, i64 ldv, int m, const dou
     }
 }
 
+// T[:, j] -= V[:, 0:m] C[:, j] with C row-major (C[i * k + j]), as the Gram products leave it
+__global__ void k_gemm_sub_rm(i64 nd, const double *V, int m, const double *C, int k, double *T) {
+    for (i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x; r < nd; r += (i64)gridDim.x * blockDim.x) {
+        for (int j = 0; j < k; ++j) {
+            double acc = 0.0;
+            for (int i = 0; i < m; ++i) acc += V[(i64)i * nd + r] * C[i * k + j];
+            T[(i64)j * nd + r] -= acc;
+        }
+    }
+}
+
 // T[:, j] = V[:, 0:m] C[:, j]
 __global__ void k_gemm_set(i64 nd, const double *V, i64 ldv, int m, const double *C, int k, double *T, i64 ldt) {
     for (i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x; r < nd; r += (i64)gridDim.x * blockDim.x) {
@@ -273,6 +284,23 @@ __global__ void k_parity(const double *src, double *dst, i64 nd, i64 nb, double 
     for (i64 J = (i64)blockIdx.x * blockDim.x + threadIdx.x; J < nd; J += (i64)gridDim.x * blockDim.x) {
         const i64 a = J / nb, b = J - a * nb;
         dst[J] = 0.5 * (src[J] + sgn * src[b * nb + a]);
+    }
+}
+
+// Start vector on the device: a perturbation amp (u - 1/2), u uniform in [0, 1)
+// from mix64(seed + r) (projected on the parity sector when sgn != 0, partner index
+// (r % nb) nb + r / nb), plus w0 at d0 and w1 at d1 (d < 0: none).
+__global__ void k_start(double *x, i64 nd, u64 seed, double amp, i64 nb, double sgn, i64 d0, double w0, i64 d1,
+                        double w1) {
+    for (i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x; r < nd; r += (i64)gridDim.x * blockDim.x) {
+        double v = amp * ((double)(mix64(seed + (u64)r) >> 11) * 0x1p-53 - 0.5);
+        if (sgn != 0.0) {
+            const i64 pr = (r % nb) * nb + r / nb;
+            v = 0.5 * (v + sgn * amp * ((double)(mix64(seed + (u64)pr) >> 11) * 0x1p-53 - 0.5));
+        }
+        if (r == d0) v += w0;
+        if (r == d1) v += w1;
+        x[r] = v;
     }
 }
 
@@ -517,11 +545,97 @@ struct Davidson {
         ++m;
     }
 
+    // Block Gram-Schmidt of the cnt corrections in V[:, m0 .. m0 + cnt): twice
+    // against V[:, 0:m0] (each pass reads the basis once for the whole block), then
+    // Cholesky-QR twice within the block; a column whose squared norm falls below
+    // 1e-20 of its value before orthogonalisation is dependent and dropped. The
+    // kept columns end orthonormal at m0 .. m0 + kept; returns kept.
+    int orthonormalize_block(int m0, int cnt) {
+        double *blk = V.p + (i64)m0 * nd;
+        const size_t ss = (size_t)cnt * cnt;
+        gram(blk, cnt, blk, cnt, dv.p + oG);  // squared norms before (diagonal)
+        PE_HIP(hipMemcpyAsync(hb.p + oG, dv.p + oG, sizeof(double) * ss, hipMemcpyDeviceToHost, s));
+        PE_HIP(hipStreamSynchronize(s));
+        std::vector<double> ref(cnt);
+        for (int j = 0; j < cnt; ++j) ref[(size_t)j] = hb.p[oG + (size_t)j * cnt + j];
+        if (m0 > 0)
+            for (int pass = 0; pass < 2; ++pass) {
+                gram(V.p, m0, blk, cnt, dv.p + oG);
+                k_gemm_sub_rm<<<g1(), 256, 0, s>>>(nd, V.p, m0, dv.p + oG, cnt, blk);
+                PE_LAUNCH_CHECK();
+            }
+        for (int round = 0; round < 2 && cnt > 0; ++round) {
+            gram(blk, cnt, blk, cnt, dv.p + oG);
+            PE_HIP(hipMemcpyAsync(hb.p + oG, dv.p + oG, sizeof(double) * (size_t)cnt * cnt, hipMemcpyDeviceToHost,
+                                  s));
+            PE_HIP(hipStreamSynchronize(s));
+            const double *S = hb.p + oG;
+            // Cholesky of S with dependent columns dropped: R (upper, over kept columns)
+            std::vector<int> keep;
+            std::vector<double> R((size_t)cnt * cnt, 0.0);
+            for (int j = 0; j < cnt; ++j) {
+                double d = S[(size_t)j * cnt + j];
+                for (size_t a = 0; a < keep.size(); ++a) {
+                    const int i = keep[a];
+                    double x = S[(size_t)i * cnt + j];
+                    for (size_t b = 0; b < a; ++b) x -= R[(size_t)keep[b] * cnt + i] * R[(size_t)keep[b] * cnt + j];
+                    x /= R[(size_t)i * cnt + i];
+                    R[(size_t)i * cnt + j] = x;
+                    d -= x * x;
+                }
+                if (d > 1e-20 * std::max(ref[(size_t)j], 1e-300) && d > 0.0) {
+                    R[(size_t)j * cnt + j] = std::sqrt(d);
+                    keep.push_back(j);
+                } else {
+                    for (int i = 0; i < cnt; ++i) R[(size_t)i * cnt + j] = 0.0;
+                }
+            }
+            // M (cnt x kept, column-major) = inverse of R restricted to the kept columns
+            const int kept = (int)keep.size();
+            std::vector<double> M((size_t)cnt * std::max(kept, 1), 0.0);
+            for (int a = 0; a < kept; ++a) {
+                // column a of R_kept^{-1}: back substitution
+                std::vector<double> z((size_t)kept, 0.0);
+                z[(size_t)a] = 1.0 / R[(size_t)keep[(size_t)a] * cnt + keep[(size_t)a]];
+                for (int b = a - 1; b >= 0; --b) {
+                    double x = 0.0;
+                    for (int c = b + 1; c <= a; ++c)
+                        x += R[(size_t)keep[(size_t)b] * cnt + keep[(size_t)c]] * z[(size_t)c];
+                    z[(size_t)b] = -x / R[(size_t)keep[(size_t)b] * cnt + keep[(size_t)b]];
+                }
+                for (int b = 0; b <= a; ++b) M[(size_t)a * cnt + keep[(size_t)b]] = z[(size_t)b];
+            }
+            std::vector<double> nref((size_t)kept);
+            for (int a = 0; a < kept; ++a) nref[(size_t)a] = ref[(size_t)keep[(size_t)a]];
+            ref.swap(nref);
+            if (kept == 0) {
+                cnt = 0;
+                break;
+            }
+            std::memcpy(hb.p + oYK, M.data(), sizeof(double) * M.size());
+            PE_HIP(hipMemcpyAsync(dv.p + oYK, hb.p + oYK, sizeof(double) * M.size(), hipMemcpyHostToDevice, s));
+            k_gemm_set<<<g1(), 256, 0, s>>>(nd, blk, nd, cnt, dv.p + oYK, kept, rowbuf.p, nd);
+            PE_LAUNCH_CHECK();
+            PE_HIP(hipMemcpyAsync(blk, rowbuf.p, sizeof(double) * nd * kept, hipMemcpyDeviceToDevice, s));
+            cnt = kept;
+        }
+        PE_HIP(hipStreamSynchronize(s));  // the host-side M must outlive its copy
+        for (int j = 0; j < cnt; ++j) {
+            hb.p[oPAIR + 2 * j] = 1.0;
+            hb.p[oPAIR + 2 * j + 1] = 1.0;
+        }
+        return cnt;
+    }
+
     // Copies the results of this iteration (residual norms; with cnt new columns
     // their G block and norm pairs) to the host and waits for them.
-    void readback(int cnt) {
+    void readback(int cnt, bool pairs = true) {
         const size_t last = cnt > 0 ? oG + (size_t)m * cnt : oRES + (size_t)k;
-        PE_HIP(hipMemcpyAsync(hb.p + oRES, dv.p + oRES, sizeof(double) * (last - oRES), hipMemcpyDeviceToHost, s));
+        if (pairs) {
+            PE_HIP(hipMemcpyAsync(hb.p + oRES, dv.p + oRES, sizeof(double) * (last - oRES), hipMemcpyDeviceToHost, s));
+        } else {
+            PE_HIP(hipMemcpyAsync(hb.p + oG, dv.p + oG, sizeof(double) * (last - oG), hipMemcpyDeviceToHost, s));
+        }
         PE_HIP(hipStreamSynchronize(s));
     }
 
@@ -731,18 +845,29 @@ struct Davidson {
                     restarted = true;
                 }
                 m0 = m;
-                int slot = 0;
-                for (int j : add) {
-                    if (m >= mmax || m >= nd) break;
-                    put_correction(j);
-                    orthonormalize_device(slot++);
-                }
-                if (m > m0) {
-                    const int cnt = m - m0;
-                    apply(m0, cnt, fp32);
-                    gram(V.p, m, W.p + (i64)m0 * nd, cnt, dv.p + oG);
-                    readback(cnt);
-                    accept_G(m0, cnt);
+                const int room = (int)std::min<i64>({(i64)add.size(), (i64)(mmax - m), nd - m});
+                if (room == 1) {
+                    put_correction(add[0]);
+                    orthonormalize_device(0);
+                    apply(m0, 1, fp32);
+                    gram(V.p, m, W.p + (i64)m0 * nd, 1, dv.p + oG);
+                    readback(1);
+                    accept_G(m0, 1);
+                } else if (room > 1) {
+                    // block Gram-Schmidt: the basis is read once per pass for all corrections
+                    for (int a = 0; a < room; ++a) {
+                        put_correction(add[(size_t)a]);
+                        ++m;
+                    }
+                    m = m0;
+                    const int kept = orthonormalize_block(m0, room);
+                    m = m0 + kept;
+                    if (kept > 0) {
+                        apply(m0, kept, fp32);
+                        gram(V.p, m, W.p + (i64)m0 * nd, kept, dv.p + oG);
+                        readback(kept, false);
+                        accept_G(m0, kept);
+                    }
                 }
             }
             if (m == m0) {
@@ -784,7 +909,7 @@ static bool solve_run(pyci_op *op, int k, const double *c0, i64 mm, i64 maxit, d
     std::partial_sort(idx.begin(), idx.begin() + want, idx.end(), [&](i64 a, i64 b) {
         return hd[(size_t)a] < hd[(size_t)b] || (hd[(size_t)a] == hd[(size_t)b] && a < b);
     });
-    std::vector<double> init((size_t)nd), noise((size_t)nd);
+    std::vector<double> init;
     std::vector<char> taken;
     if (sector != 0) taken.assign((size_t)nd, 0);
     size_t cursor = 0;
@@ -802,42 +927,40 @@ static bool solve_run(pyci_op *op, int k, const double *c0, i64 mm, i64 maxit, d
     }
     for (int j = 0; D.m < kb0 && j < kb0 + 64; ++j) {
         bool have = false;
+        double *col = D.V.p + (i64)D.m * nd;
         if (j == 0 && use_c0) {
-            std::memcpy(init.data(), c0, sizeof(double) * nd);
+            init.assign(c0, c0 + nd);
             if (sector != 0)
                 for (i64 r = 0; r < nd; ++r) init[(size_t)r] = 0.5 * (c0[r] + sector * c0[partner(r)]);
+            PE_HIP(hipMemcpyAsync(col, init.data(), sizeof(double) * nd, hipMemcpyHostToDevice, s));
             have = true;
         } else {
             // perturbation of total norm ~1e-3: seeds every direction without moving
             // the start away from the dominant determinant (a perturbation of norm
             // ~0.6 leaves a start whose Rayleigh quotient is far above the lowest
-            // diagonal element and can converge to an excited root)
-            u64 seed = 0x243f6a8885a308d3ULL + (u64)j;
+            // diagonal element and can converge to an excited root); built on the device
+            const u64 seed = 0x243f6a8885a308d3ULL + (u64)j;
             const double amp = 2e-3 * std::sqrt(3.0 / (double)nd);
-            for (i64 r = 0; r < nd; ++r) noise[(size_t)r] = amp * ((double)(mix64(seed + (u64)r) >> 11) * 0x1p-53 - 0.5);
             for (; cursor < (size_t)want && !have; ++cursor) {
                 const i64 d0 = idx[cursor];
                 if (sector == 0) {
-                    init = noise;
-                    init[(size_t)d0] += 1.0;
+                    k_start<<<D.g1(), 256, 0, s>>>(col, nd, seed, amp, 1, 0.0, d0, 1.0, -1, 0.0);
                     have = true;
                     continue;
                 }
                 const i64 d1 = partner(d0);
                 if (taken[(size_t)d0] || (d1 == d0 && sector < 0)) continue;
                 taken[(size_t)d0] = taken[(size_t)d1] = 1;
-                for (i64 r = 0; r < nd; ++r) init[(size_t)r] = 0.5 * (noise[(size_t)r] + sector * noise[(size_t)partner(r)]);
-                if (d1 == d0) {
-                    init[(size_t)d0] += 1.0;
-                } else {
-                    init[(size_t)d0] += std::sqrt(0.5);
-                    init[(size_t)d1] += sector * std::sqrt(0.5);
-                }
+                if (d1 == d0)
+                    k_start<<<D.g1(), 256, 0, s>>>(col, nd, seed, amp, nb, (double)sector, d0, 1.0, -1, 0.0);
+                else
+                    k_start<<<D.g1(), 256, 0, s>>>(col, nd, seed, amp, nb, (double)sector, d0, std::sqrt(0.5), d1,
+                                                   sector * std::sqrt(0.5));
                 have = true;
             }
+            if (have) PE_LAUNCH_CHECK();
         }
         if (!have) continue;
-        PE_HIP(hipMemcpyAsync(D.V.p + (i64)D.m * nd, init.data(), sizeof(double) * nd, hipMemcpyHostToDevice, s));
         std::vector<double> n0 = D.gram_host(D.V.p + (i64)D.m * nd, 1, D.V.p + (i64)D.m * nd, 1);
         if (std::sqrt(std::max(n0[0], 0.0)) > 1e-12 && D.orthonormalize_new(std::sqrt(std::max(n0[0], 0.0)))) ++D.m;
     }

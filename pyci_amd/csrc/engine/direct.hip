@@ -123,6 +123,7 @@ __host__ __device__ constexpr int dprod_vc(int MT) {
 
 struct DprodArgs {
     int ncol, NB, nvec, npair, nea, ntot;
+    int vstride, v0;  // X[J][v] of vector v0 + v (v < nvec) at X[J * vstride + v0 + v]
     int offb;  // first beta row of the merged table, NA * nea
     int sent;  // index of the sentinel entry
     int ld;
@@ -153,9 +154,9 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
     // The two roles run separate loops with the same barrier count, so that neither
     // role's registers are live in the other.
     if (wave >= 4) {
-        const int g = wave - 4, sa = A.NB * A.nvec;
+        const int g = wave - 4, sa = A.NB * A.vstride;
         // loads the table entries of the tile of `step` into nx and its X bases: the X
-        // index of an alpha entry is en.x * sa + na, of a beta entry en.x * nvec + nb
+        // index of an alpha entry is en.x * sa + na, of a beta entry en.x * vstride + nb
         auto fetch = [&](int step, int2 *nx, int &na, int &nb) {
             const int c0 = ((int)blockIdx.x + step * (int)gridDim.x) * 64 + lane;
             const bool live = step < nmine && c0 < A.ncol;
@@ -163,8 +164,8 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
             const unsigned J = c / (unsigned)A.nvec, v = c - J * (unsigned)A.nvec;
             const unsigned Ja = J / (unsigned)A.NB, Jb = J - Ja * (unsigned)A.NB;
             const int ra = (int)(Ja * (unsigned)A.nea), rb = A.offb + (int)(Jb * (unsigned)(A.ntot - A.nea)) - A.nea;
-            na = (int)(Jb * (unsigned)A.nvec + v);
-            nb = (int)(Ja * (unsigned)sa + v);
+            na = (int)(Jb * (unsigned)A.vstride + v) + A.v0;
+            nb = (int)(Ja * (unsigned)sa + v) + A.v0;
 #pragma unroll
             for (int u = 0; u < PE; ++u) {
                 const int t = g + 8 * u;
@@ -176,7 +177,7 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
 #pragma unroll
             for (int u = 0; u < PE; ++u) {
                 const int t = g + 8 * u;
-                val[u] = X[(t < A.nea) ? en[u].x * sa + xa : en[u].x * A.nvec + xb];
+                val[u] = X[(t < A.nea) ? en[u].x * sa + xa : en[u].x * A.vstride + xb];
             }
         };
         auto add = [&](const int2 *en, T *val, double *buf) {
@@ -227,8 +228,12 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
             if constexpr (VM == VM_FULL) return vs[(16 * m + lr) * LS + 4 * q + lk];
             else return V[(16 * m + lr) * KP + 4 * q + lk];
         };
-        // X[c] of the tile multiplied next, loaded one tile ahead
-        T x = X[min((int)blockIdx.x * 64 + 16 * wave + lr, A.ncol - 1)];
+        // X of column c (determinant c / nvec, vector c % nvec), loaded one tile ahead
+        auto xat = [&](int c) -> T {
+            const int cc = min(c, A.ncol - 1), J = cc / A.nvec;
+            return X[J * A.vstride + A.v0 + (cc - J * A.nvec)];
+        };
+        T x = xat((int)blockIdx.x * 64 + 16 * wave + lr);
         auto product = [&](int tile, double *buf) {
             const int c = tile * 64 + 16 * wave + lr;
             double *bb = buf + 16 * wave + lr;
@@ -277,7 +282,7 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
                         w[p * 64] = (T)0.5 * acc[m][r] + ks[p] * x;
                     }
             }
-            x = X[min(c + 64 * (int)gridDim.x, A.ncol - 1)];
+            x = xat(c + 64 * (int)gridDim.x);
         };
         if constexpr (!SB) {
             __syncthreads();
@@ -301,7 +306,7 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
 template <int NV, typename T, bool TILED>
 __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int neb, int KP,
                                                   const int2 *__restrict__ ta, const int2 *__restrict__ tb, int nvec,
-                                                  const T *__restrict__ W, T *__restrict__ Y) {
+                                                  int vstride, int v0, const T *__restrict__ W, T *__restrict__ Y) {
     i64 Ia, Ib;
     if (!det_of<TILED>(nd, nd / NB, NB, Ia, Ib)) return;
     const i64 I = Ia * NB + Ib;
@@ -326,7 +331,7 @@ __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int n
         for (int v = 0; v < NV; ++v)
             if (v < nvec) acc[v] += s * W[wt_at(c + v, e.y & 0xffff, KP)];
     }
-    T *y = Y + I * nvec;
+    T *y = Y + I * vstride + v0;
 #pragma unroll
     for (int v = 0; v < NV; ++v)
         if (v < nvec) y[v] = acc[v];
@@ -553,28 +558,16 @@ void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn) {
     op->direct = std::move(D);
 }
 
-// Workspace of the E values: ceil(nd nvec / 64) tiles of kpad x 64 (tile-major);
-// grown on demand, never cleared (the scatter pass reads only what the product
-// pass of the same sigma wrote).
+// Workspace of the E values: ceil(columns / 64) tiles of kpad x 64 (tile-major),
+// held by the context and shared by its operators (their sigmas run in order on
+// the context's stream); grown on demand, never cleared (the scatter pass reads
+// only what the product pass of the same sigma wrote).
 template <typename T>
-static T *direct_workspace(DirectOp *D, i64 cols);
-
-template <>
-double *direct_workspace<double>(DirectOp *D, i64 cols) {
-    if (D->ldw64 < cols) {
-        D->work64.alloc((size_t)D->kpad * cols);
-        D->ldw64 = cols;
-    }
-    return D->work64.p;
-}
-
-template <>
-float *direct_workspace<float>(DirectOp *D, i64 cols) {
-    if (D->ldw32 < cols) {
-        D->work32.alloc((size_t)D->kpad * cols);
-        D->ldw32 = cols;
-    }
-    return D->work32.p;
+static T *direct_workspace(const pyci_op *op, i64 elems) {
+    pyci_ctx *ctx = op->ctx;
+    const size_t bytes = sizeof(T) * (size_t)elems;
+    if (ctx->mf_ws.n < bytes) ctx->mf_ws.alloc(bytes);
+    return reinterpret_cast<T *>(ctx->mf_ws.p);
 }
 
 // LDS one block may use (160 KB on gfx950) and compute units (256), from the current device.
@@ -624,8 +617,8 @@ struct DprodPlan {
 };
 
 template <int MT, typename T>
-static void launch_dprod(const DirectOp *D, i64 nd, int nvec, const T *X, const T *V, const T *kv, T *W,
-                         hipStream_t s) {
+static void launch_dprod(const DirectOp *D, i64 nd, int nvec, int vstride, int v0, const T *X, const T *V,
+                         const T *kv, T *W, hipStream_t s) {
     typedef DprodPlan<MT, T> P;
     static const size_t cap = lds_per_block();
     const int ntot = D->nea + D->neb;
@@ -635,7 +628,8 @@ static void launch_dprod(const DirectOp *D, i64 nd, int nvec, const T *X, const 
     PE_REQUIRE(!fixed_npair(MT) || D->npair == fixed_npair(MT), PYCI_EUNSUPPORTED,
                "pair count does not match the pair dimension");
     PE_REQUIRE(P::lds <= cap, PYCI_EUNSUPPORTED, "the device has too little LDS for the matrix-free product");
-    DprodArgs A{(int)(nd * nvec), (int)D->nbs, nvec, D->npair, D->nea, ntot, (int)(D->na * D->nea),
+    PE_REQUIRE(nd * vstride < ((i64)1 << 31), PYCI_EUNSUPPORTED, "too many columns for the matrix-free operator");
+    DprodArgs A{(int)(nd * nvec), (int)D->nbs, nvec, D->npair, D->nea, ntot, vstride, v0, (int)(D->na * D->nea),
                 (int)(D->na * D->nea + D->nbs * D->neb), P::ld};
     if (ntot <= 64) launch_dprod_v<MT, T, P::SB, P::VM, 8>(D, A, X, V, kv, W, P::lds, s);
     else if (ntot <= 96) launch_dprod_v<MT, T, P::SB, P::VM, 12>(D, A, X, V, kv, W, P::lds, s);
@@ -644,18 +638,18 @@ static void launch_dprod(const DirectOp *D, i64 nd, int nvec, const T *X, const 
 
 // ev (optional): 3 events recorded before, between and after the two passes.
 template <int NV, typename T>
-static void launch_passes(const DirectOp *D, i64 nd, int nvec, const T *X, T *W, T *Y, const T *V, const T *kv,
-                          hipStream_t s, hipEvent_t *ev) {
+static void launch_passes(const DirectOp *D, i64 nd, int nvec, int vstride, int v0, const T *X, T *W, T *Y,
+                          const T *V, const T *kv, hipStream_t s, hipEvent_t *ev) {
     if (ev) PE_HIP(hipEventRecord(ev[0], s));
     switch (D->kpad / 16) {
-        case 1: launch_dprod<1, T>(D, nd, nvec, X, V, kv, W, s); break;
-        case 2: launch_dprod<2, T>(D, nd, nvec, X, V, kv, W, s); break;
-        case 3: launch_dprod<3, T>(D, nd, nvec, X, V, kv, W, s); break;
-        case 4: launch_dprod<4, T>(D, nd, nvec, X, V, kv, W, s); break;
-        case 5: launch_dprod<5, T>(D, nd, nvec, X, V, kv, W, s); break;
-        case 6: launch_dprod<6, T>(D, nd, nvec, X, V, kv, W, s); break;
-        case 7: launch_dprod<7, T>(D, nd, nvec, X, V, kv, W, s); break;
-        case 8: launch_dprod<8, T>(D, nd, nvec, X, V, kv, W, s); break;
+        case 1: launch_dprod<1, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
+        case 2: launch_dprod<2, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
+        case 3: launch_dprod<3, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
+        case 4: launch_dprod<4, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
+        case 5: launch_dprod<5, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
+        case 6: launch_dprod<6, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
+        case 7: launch_dprod<7, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
+        case 8: launch_dprod<8, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
         default: throw EngineError(PYCI_EUNSUPPORTED, "pair dimension too large for the matrix-free operator");
     }
     if (ev) PE_HIP(hipEventRecord(ev[1], s));
@@ -663,27 +657,35 @@ static void launch_passes(const DirectOp *D, i64 nd, int nvec, const T *X, T *W,
     const unsigned g = tiled ? tiled_grid(D->na, D->nbs) : grid_for(nd, 256);
     if (tiled)
         k_mf_sigma<NV, T, true><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->kpad, D->tab_a.p, D->tab_b.p, nvec,
-                                                  W, Y);
+                                                  vstride, v0, W, Y);
     else
         k_mf_sigma<NV, T, false><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->kpad, D->tab_a.p, D->tab_b.p,
-                                                   nvec, W, Y);
+                                                   nvec, vstride, v0, W, Y);
     PE_LAUNCH_CHECK();
     if (ev) PE_HIP(hipEventRecord(ev[2], s));
 }
 
+// The vectors go through the two passes in chunks whose E workspace stays within
+// 16 GB (one vector at least): at nbasis 14 one FP64 vector already needs 10.6 GB.
 template <typename T>
 void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, hipEvent_t *ev) {
     DirectOp *D = op->direct.get();
     PE_REQUIRE(nvec >= 1 && nvec <= 16, PYCI_EUNSUPPORTED, "the matrix-free operator takes 1..16 vectors");
     const i64 nd = op->nrow;
-    T *W = direct_workspace<T>(D, (nd * nvec + 63) / 64 * 64);
+    const i64 per_vec = ((nd + 63) / 64 * 64) * (i64)D->kpad * (i64)sizeof(T);
+    const int chunk = (int)std::max<i64>(1, std::min<i64>(nvec, ((i64)16 << 30) / per_vec));
+    T *W = direct_workspace<T>(op, ((nd * chunk + 63) / 64 * 64) * (i64)D->kpad);
     const T *V = std::is_same<T, double>::value ? (const T *)D->V64.p : (const T *)D->V32.p;
     const T *kv = std::is_same<T, double>::value ? (const T *)D->k64.p : (const T *)D->k32.p;
-    if (nvec == 1) launch_passes<1, T>(D, nd, nvec, X, W, Y, V, kv, s, ev);
-    else if (nvec == 2) launch_passes<2, T>(D, nd, nvec, X, W, Y, V, kv, s, ev);
-    else if (nvec <= 4) launch_passes<4, T>(D, nd, nvec, X, W, Y, V, kv, s, ev);
-    else if (nvec <= 8) launch_passes<8, T>(D, nd, nvec, X, W, Y, V, kv, s, ev);
-    else launch_passes<16, T>(D, nd, nvec, X, W, Y, V, kv, s, ev);
+    for (int v0 = 0; v0 < nvec; v0 += chunk) {
+        const int nc = std::min(chunk, nvec - v0);
+        hipEvent_t *e = (ev && v0 == 0 && nc == nvec) ? ev : nullptr;
+        if (nc == 1) launch_passes<1, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e);
+        else if (nc == 2) launch_passes<2, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e);
+        else if (nc <= 4) launch_passes<4, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e);
+        else if (nc <= 8) launch_passes<8, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e);
+        else launch_passes<16, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e);
+    }
 }
 
 template void direct_sigma<double>(const pyci_op *, int, const double *, double *, hipStream_t, hipEvent_t *);
@@ -696,7 +698,12 @@ template void direct_sigma<float>(const pyci_op *, int, const float *, float *, 
 void direct_bench_parts(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *ms, double *bytes, double *flops) {
     DirectOp *D = op->direct.get();
     hipStream_t s = op->ctx->stream;
-    const i64 nd = op->nrow, N = nd * nvec;
+    const i64 nd = op->nrow;
+    // one chunk of vectors (direct_sigma's 16 GB workspace budget), so that the
+    // events bracket exactly the two passes
+    const i64 per_vec = ((nd + 63) / 64 * 64) * (i64)D->kpad * (fp32 ? 4 : 8);
+    nvec = std::max<i64>(1, std::min<i64>(nvec, ((i64)16 << 30) / per_vec));
+    const i64 N = nd * nvec;
     DevBuf<double> x((size_t)N), y((size_t)N);
     DevBuf<float> x32, y32;
     k_direct_fill<<<1024, 256, 0, s>>>(x.p, N, 12345);

@@ -129,6 +129,7 @@ struct pyci_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     pe::DevBuf<pe::i64> binom;  // C(n, k) for 0 <= n, k <= 64, row-major [n][k] (65 x 65)
+    pe::DevBuf<unsigned char> mf_ws;  // E workspace of the matrix-free sigma (shared by the operators)
 };
 
 struct pyci_ham {
@@ -166,9 +167,6 @@ struct DirectOp {
     DevBuf<u64> str_a, str_b;    // strings in colex order
     DevBuf<double> one_mo, two_mo, V64, k64;
     DevBuf<float> V32, k32;
-    DevBuf<double> work64;       // E values, tile-major: ceil(columns / 64) tiles of kpad x 64
-    DevBuf<float> work32;
-    i64 ldw64 = 0, ldw32 = 0;    // columns the workspaces hold (multiple of 64)
 };
 }  // namespace pe
 
