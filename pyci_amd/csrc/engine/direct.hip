@@ -263,9 +263,13 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
 #pragma unroll
                         for (int m = 0; m < MTL; ++m) an[m] = aload(m, q + 1);
                     }
+                    // the loads of step q + 1 go out before the MFMAs of step q, and the
+                    // loads of later steps after them (no instruction crosses either point;
+                    // not in the one-buffer plans, whose register budget it would exceed)
+                    if constexpr (!SB) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int m = 0; m < MTL; ++m) acc[m] = mfma16(ac[m], bc, acc[m]);
-                    // keeps the fragment loads of later steps from being hoisted above
+                    if constexpr (!SB) __builtin_amdgcn_sched_barrier(0);
                     asm volatile("" ::: "memory");
 #pragma unroll
                     for (int m = 0; m < MTL; ++m) ac[m] = an[m];
