@@ -464,16 +464,16 @@ struct Davidson {
             }
             k_col2row<double, float><<<g1(), 256, 0, s>>>(nd, cnt, src, nd, row32.p);
             PE_LAUNCH_CHECK();
-            sigma_f32(op, cnt, row32.p, out32.p, s);
+            sigma_f32(op, cnt, row32.p, out32.p, s, sector);
             k_row2col<float, double><<<g1(), 256, 0, s>>>(nd, cnt, out32.p, dst, nd);
             PE_LAUNCH_CHECK();
             st.sigma_calls_f32 += cnt;
         } else if (cnt == 1) {
-            sigma_f64(op, 1, src, dst, s);
+            sigma_f64(op, 1, src, dst, s, sector);
         } else {
             k_col2row<double, double><<<g1(), 256, 0, s>>>(nd, cnt, src, nd, rowbuf.p);
             PE_LAUNCH_CHECK();
-            sigma_f64(op, cnt, rowbuf.p, rowout.p, s);
+            sigma_f64(op, cnt, rowbuf.p, rowout.p, s, sector);
             k_row2col<double, double><<<g1(), 256, 0, s>>>(nd, cnt, rowout.p, dst, nd);
             PE_LAUNCH_CHECK();
         }

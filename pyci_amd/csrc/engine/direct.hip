@@ -307,10 +307,12 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
 }
 
 // Scatter pass: Y[I][v] = sum over the alpha and beta table entries of I of s E[pair][K v].
+// With alpha_only the beta terms are left out (see k_mf_symm).
 template <int NV, typename T, bool TILED>
 __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int neb, int KP,
                                                   const int2 *__restrict__ ta, const int2 *__restrict__ tb, int nvec,
-                                                  int vstride, int v0, const T *__restrict__ W, T *__restrict__ Y) {
+                                                  int vstride, int v0, int alpha_only, const T *__restrict__ W,
+                                                  T *__restrict__ Y) {
     i64 Ia, Ib;
     if (!det_of<TILED>(nd, nd / NB, NB, Ia, Ib)) return;
     const i64 I = Ia * NB + Ib;
@@ -327,7 +329,7 @@ __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int n
             if (v < nvec) acc[v] += s * W[wt_at(c + v, e.y & 0xffff, KP)];
     }
     const int2 *pb = tb + Ib * neb;
-    for (int t = 0; t < neb; ++t) {
+    for (int t = 0; t < (alpha_only ? 0 : neb); ++t) {
         const int2 e = pb[t];
         const T s = (e.y >> 16) ? (T)-1 : (T)1;
         const i64 c = (Ia * NB + e.x) * nvec;
@@ -339,6 +341,31 @@ __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int n
 #pragma unroll
     for (int v = 0; v < NV; ++v)
         if (v < nvec) y[v] = acc[v];
+}
+
+// Sigma of a vector in a parity sector of the alpha <-> beta exchange P (nup == ndn,
+// J = (Ja, Jb) -> (Jb, Ja); x(P J) = s x(J)): D and E then have the same parity,
+// E[p][(Ia, K)] = s E[p][(K, Ia)], so the beta terms of determinant I equal s times
+// the alpha terms of P(I) (the alpha and beta string tables are the same) and
+//   y(I) = A(I) + s A(P I),   A = the alpha terms (k_mf_sigma with alpha_only).
+// In place on Y: the thread of I with Ia <= Ib owns the pair (I, P I).
+template <typename T>
+__global__ void k_mf_symm(T *Y, i64 NB, int nvec, int vstride, int v0, T s) {
+    const i64 nd = NB * NB;
+    for (i64 I = (i64)blockIdx.x * blockDim.x + threadIdx.x; I < nd; I += (i64)gridDim.x * blockDim.x) {
+        const i64 Ia = I / NB, Ib = I - Ia * NB;
+        if (Ia > Ib) continue;
+        T *y0 = Y + I * vstride + v0, *y1 = Y + (Ib * NB + Ia) * vstride + v0;
+        for (int v = 0; v < nvec; ++v) {
+            const T a = y0[v], b = y1[v];
+            if (Ia == Ib) {
+                y0[v] = a + s * a;
+            } else {
+                y0[v] = a + s * b;
+                y1[v] = b + s * a;
+            }
+        }
+    }
 }
 
 __global__ void k_direct_fill(double *x, i64 n, u64 seed) {
@@ -643,7 +670,7 @@ static void launch_dprod(const DirectOp *D, i64 nd, int nvec, int vstride, int v
 // ev (optional): 3 events recorded before, between and after the two passes.
 template <int NV, typename T>
 static void launch_passes(const DirectOp *D, i64 nd, int nvec, int vstride, int v0, const T *X, T *W, T *Y,
-                          const T *V, const T *kv, hipStream_t s, hipEvent_t *ev) {
+                          const T *V, const T *kv, hipStream_t s, hipEvent_t *ev, int sector) {
     if (ev) PE_HIP(hipEventRecord(ev[0], s));
     switch (D->kpad / 16) {
         case 1: launch_dprod<1, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
@@ -657,23 +684,39 @@ static void launch_passes(const DirectOp *D, i64 nd, int nvec, int vstride, int 
         default: throw EngineError(PYCI_EUNSUPPORTED, "pair dimension too large for the matrix-free operator");
     }
     if (ev) PE_HIP(hipEventRecord(ev[1], s));
+    if (sector != 0) {
+        // alpha terms (every block of 256 consecutive determinants shares few alpha
+        // strings, so their table rows are common and the E reads contiguous), then
+        // the parity symmetrisation
+        k_mf_sigma<NV, T, false><<<grid_for(nd, 256), 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->kpad, D->tab_a.p,
+                                                                   D->tab_b.p, nvec, vstride, v0, 1, W, Y);
+        PE_LAUNCH_CHECK();
+        k_mf_symm<T><<<grid_for(nd, 256), 256, 0, s>>>(Y, D->nbs, nvec, vstride, v0, (T)sector);
+        PE_LAUNCH_CHECK();
+        if (ev) PE_HIP(hipEventRecord(ev[2], s));
+        return;
+    }
     const bool tiled = (D->nbs + 15) / 16 >= 16;  // enough column tiles to spread over the XCDs
     const unsigned g = tiled ? tiled_grid(D->na, D->nbs) : grid_for(nd, 256);
     if (tiled)
         k_mf_sigma<NV, T, true><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->kpad, D->tab_a.p, D->tab_b.p, nvec,
-                                                  vstride, v0, W, Y);
+                                                  vstride, v0, 0, W, Y);
     else
         k_mf_sigma<NV, T, false><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->kpad, D->tab_a.p, D->tab_b.p,
-                                                   nvec, vstride, v0, W, Y);
+                                                   nvec, vstride, v0, 0, W, Y);
     PE_LAUNCH_CHECK();
     if (ev) PE_HIP(hipEventRecord(ev[2], s));
 }
 
 // The vectors go through the two passes in chunks whose E workspace stays within
 // 16 GB (one vector at least): at nbasis 14 one FP64 vector already needs 10.6 GB.
+// sector = +1 / -1: every vector lies in that parity sector of the alpha <-> beta
+// exchange (only where nup == ndn; the solver's sector runs pass it), which halves
+// the scatter pass (k_mf_symm); 0: any vectors.
 template <typename T>
-void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, hipEvent_t *ev) {
+void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, hipEvent_t *ev, int sector) {
     DirectOp *D = op->direct.get();
+    if (D->nup != D->ndn) sector = 0;
     PE_REQUIRE(nvec >= 1 && nvec <= 16, PYCI_EUNSUPPORTED, "the matrix-free operator takes 1..16 vectors");
     const i64 nd = op->nrow;
     const i64 per_vec = ((nd + 63) / 64 * 64) * (i64)D->kpad * (i64)sizeof(T);
@@ -684,16 +727,16 @@ void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, 
     for (int v0 = 0; v0 < nvec; v0 += chunk) {
         const int nc = std::min(chunk, nvec - v0);
         hipEvent_t *e = (ev && v0 == 0 && nc == nvec) ? ev : nullptr;
-        if (nc == 1) launch_passes<1, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e);
-        else if (nc == 2) launch_passes<2, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e);
-        else if (nc <= 4) launch_passes<4, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e);
-        else if (nc <= 8) launch_passes<8, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e);
-        else launch_passes<16, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e);
+        if (nc == 1) launch_passes<1, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e, sector);
+        else if (nc == 2) launch_passes<2, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e, sector);
+        else if (nc <= 4) launch_passes<4, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e, sector);
+        else if (nc <= 8) launch_passes<8, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e, sector);
+        else launch_passes<16, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e, sector);
     }
 }
 
-template void direct_sigma<double>(const pyci_op *, int, const double *, double *, hipStream_t, hipEvent_t *);
-template void direct_sigma<float>(const pyci_op *, int, const float *, float *, hipStream_t, hipEvent_t *);
+template void direct_sigma<double>(const pyci_op *, int, const double *, double *, hipStream_t, hipEvent_t *, int);
+template void direct_sigma<float>(const pyci_op *, int, const float *, float *, hipStream_t, hipEvent_t *, int);
 
 // Per-pass timing of nrep matrix-free sigmas on random X (see pyci_op_bench_sigma_parts).
 // Algorithmic bytes and flops per launch, with N = nd * nvec columns and T the value size:

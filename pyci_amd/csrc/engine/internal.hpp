@@ -315,8 +315,10 @@ void build_rows(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn, i64 row0,
                 DevBuf<i64> &rowptr_seg, DevBuf<int32_t> &cols, DevBuf<double> &vals, DevBuf<i64> &lowcnt_seg,
                 DevBuf<double> &diag_seg, i64 &nnz_seg, i64 &low_total);
 // SpMM (sigma.hip): Y = A X, X ncol x nvec row-major, Y nrow x nvec row-major
-void sigma_f64(const pyci_op *op, int nvec, const double *X, double *Y, hipStream_t s);
-void sigma_f32(const pyci_op *op, int nvec, const float *X, float *Y, hipStream_t s);
+// sector = +1 / -1: the vectors lie in that parity sector of the alpha <-> beta
+// exchange (a hint the matrix-free operator uses; 0 = any vectors)
+void sigma_f64(const pyci_op *op, int nvec, const double *X, double *Y, hipStream_t s, int sector = 0);
+void sigma_f32(const pyci_op *op, int nvec, const float *X, float *Y, hipStream_t s, int sector = 0);
 void ensure_vals32(pyci_op *op);
 // Davidson (davidson.hip)
 void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, double tol, int precision,
@@ -343,7 +345,8 @@ void hci_pool_device(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double eps,
 bool direct_applicable(const pyci_ham *ham, const pyci_wfn *wfn, i64 nrow, i64 ncol, int symmetric);
 void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn);
 template <typename T>
-void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, hipEvent_t *ev = nullptr);
+void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, hipEvent_t *ev = nullptr,
+                  int sector = 0);
 void direct_bench_parts(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *ms, double *bytes, double *flops);
 void direct_ensure_csr(pyci_op *op);
 // helpers shared with the batched solver (direct.hip)

@@ -47,12 +47,15 @@ def test_direct_default_and_exports_are_the_csr():
     np.testing.assert_array_equal(b.data(), a.data())
 
 
+# (8, 3, 3): nup == ndn, so the solver runs both alpha-beta exchange sectors and the
+# sector-symmetric scatter (beta terms from the alpha terms of the exchanged determinant)
+@pytest.mark.parametrize("nb,nu,nd", [(8, 4, 3), (8, 3, 3)])
 @pytest.mark.parametrize("precision", ["f64", "mixed"])
 @pytest.mark.parametrize("n", [1, 3])
-def test_direct_solve_matches_oracle(oracle, precision, n):
-    ham, wfn, one, two = fullci(8, 4, 3, seed=9)
-    dets = oracle.all_dets("fullci", 8, 4, 3)
-    ref = oracle.lowest_eigs(oracle.csr_build("fullci", 8, 4, 3, dets, one, two), n)[0]
+def test_direct_solve_matches_oracle(oracle, precision, n, nb, nu, nd):
+    ham, wfn, one, two = fullci(nb, nu, nd, seed=9)
+    dets = oracle.all_dets("fullci", nb, nu, nd)
+    ref = oracle.lowest_eigs(oracle.csr_build("fullci", nb, nu, nd, dets, one, two), n)[0]
     op = pyci.sparse_op(ham, wfn)
     es, cs = op.solve(n=n, tol=1e-10, precision=precision)
     np.testing.assert_allclose(np.sort(es), ref, rtol=0, atol=1e-10)
