@@ -518,7 +518,7 @@ struct Davidson {
         const double *p0 = pb.p + pN0;
         double *p1 = pb.p + pN1, *p2 = pb.p + pN2;
         if (m == 0) {
-            k_scale_final<<<g1(), DT, 0, s>>>(col, nd, p0, p0, p0, nblk, 0, pair);
+            k_scale_final<<<(unsigned)nblk, DT, 0, s>>>(col, nd, p0, p0, p0, nblk, 0, pair);
         } else if (m <= kCgsMax) {
             const dim3 grid((unsigned)nblk, (unsigned)((m + 7) / 8));
             k_gram<<<grid, DT, 0, s>>>(nd, V.p, nd, m, col, nd, 1, pb.p + pC1, nullptr, nullptr);
@@ -529,7 +529,7 @@ struct Davidson {
             PE_LAUNCH_CHECK();
             k_update_norm<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, m, pb.p + pC2, col, p2, p0, p1);
             PE_LAUNCH_CHECK();
-            k_scale_final<<<g1(), DT, 0, s>>>(col, nd, p0, p1, p2, nblk, 1, pair);
+            k_scale_final<<<(unsigned)nblk, DT, 0, s>>>(col, nd, p0, p1, p2, nblk, 1, pair);
         } else {
             // large bases: both passes unconditionally, coefficients reduced apart
             for (int pass = 0; pass < 2; ++pass) {
@@ -539,7 +539,7 @@ struct Davidson {
             }
             k_put<<<(unsigned)nblk, DT, 0, s>>>(col, col, nd, swap_nb, 0.0, p1);
             PE_LAUNCH_CHECK();
-            k_scale_final<<<g1(), DT, 0, s>>>(col, nd, p0, p1, p1, nblk, 0, pair);
+            k_scale_final<<<(unsigned)nblk, DT, 0, s>>>(col, nd, p0, p1, p1, nblk, 0, pair);
         }
         PE_LAUNCH_CHECK();
         ++m;
@@ -1002,10 +1002,11 @@ void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, 
     const i64 nd = op->nrow;
     const int k = (int)n;
     PE_REQUIRE(k <= 16, PYCI_EUNSUPPORTED, "solve supports at most 16 roots per call");
-    // default subspace max(10, 2n+1, 6n): thick restarts keep 2n Ritz vectors, so a
-    // single root converges as fast as with the reference's Lanczos default of 20
-    // while each iteration streams half the basis; blocks get 6n (fewer restarts)
-    i64 mm = (ncv > 0) ? ncv : std::min<i64>(nd, std::max<i64>({2 * n + 1, 10, 6 * n}));
+    // default subspace max(14, 2n+1, 6n): thick restarts keep 2n Ritz vectors, so a
+    // single root converges about as fast as with the reference's Lanczos default of
+    // 20 while each iteration streams fewer basis vectors (FullCI nbasis 12, one root:
+    // 131 / 125 / 125 / 135 ms with 10 / 14 / 20 / 30); blocks get 6n (fewer restarts)
+    i64 mm = (ncv > 0) ? ncv : std::min<i64>(nd, std::max<i64>({2 * n + 1, 14, 6 * n}));
     mm = std::max<i64>(mm, 2 * n + 1);
     mm = std::min<i64>(mm, nd);
     mm = std::max<i64>(mm, n);
