@@ -53,6 +53,7 @@ struct BProb {
     i64 swap_nb;       // sector != 0: beta-string count of the alpha <-> beta exchange P
     int sector;        // 0, or +1 / -1: this sub-problem's parity sector of P
     int orig;          // the caller's problem index
+    int big;           // sigma by the fused pass of direct.hip (own launches) instead of k_bmf_*
 };
 
 __device__ inline i64 swap_partner(i64 J, i64 nb) { return (J % nb) * nb + J / nb; }
@@ -767,7 +768,37 @@ struct Batch {
     DevBuf<unsigned long long> tconv;
     i64 nrowblk = 0, ncolblk = 0, ntile = 0;
     BView view{};
+    // large sub-problems: the fused matrix-free pass (direct.hip), one launch pair per
+    // sub-problem and precision in the captured iteration, gated by a device descriptor
+    std::vector<int> bigp;     // sub-problem indices
+    std::vector<MfView> bigv;  // their operator views
+    DevBuf<int> bigp_d;
+    DevBuf<int2> mtab;         // merged tables with sentinels
+    DevBuf<long long> dyn;     // per big sub-problem: {nvec, X, Y} FP64, then FP32
+    DevBuf<unsigned char> fws; // shared E workspace of the fused pass
 };
+
+// Sub-problems with at least this many determinants use the fused pass.
+constexpr i64 kBigND = 65536;
+
+// The device descriptors of the fused-pass launches: the new columns of an active
+// sub-problem in the precision of its phase (offsets into V / W, or the FP32 copies).
+__global__ void k_btask(BView b, const int *bigp, int nbig, long long *dyn) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nbig) return;
+    const int p = bigp[q];
+    const int *st = b.st + p * ST;
+    const BProb P = b.prob[p];
+    const bool on = st[S_ACTIVE] && st[S_NADD] > 0;
+    long long *d64 = dyn + 6 * q, *d32 = d64 + 3;
+    const long long off = P.basis + (long long)st[S_C0] * P.nd;
+    d64[0] = (on && st[S_PHASE] == 0) ? st[S_NADD] : 0;
+    d64[1] = off;
+    d64[2] = off;
+    d32[0] = (on && st[S_PHASE] == 1) ? st[S_NADD] : 0;
+    d32[1] = P.s32;
+    d32[2] = P.s32;
+}
 
 static void build_row_map(Batch &B, int RB) {
     std::vector<int> blk;
@@ -829,21 +860,41 @@ static void orthonormalize_and_apply(Batch &B, hipStream_t s) {
     k_bchol<<<(np + 63) / 64, 64, 0, s>>>(v, 0);
     PE_LAUNCH_CHECK();
     launch_rows(B, k_bapplyR, s);
-    // sigma of the new columns: FP64 problems, then FP32 ones
-    k_bmf_d<double><<<(unsigned)B.ncolblk, 256, 0, s>>>(v);
-    PE_LAUNCH_CHECK();
-    k_bmf_gemm<double><<<(unsigned)B.ntile, 256, 0, s>>>(v);
-    PE_LAUNCH_CHECK();
-    k_bmf_sigma<double><<<(unsigned)B.ncolblk, 256, 0, s>>>(v);
-    PE_LAUNCH_CHECK();
+    // sigma of the new columns: FP64 problems, then FP32 ones; small sub-problems in
+    // the batched kernels (one launch each for all), large ones by the fused pass
+    const int nbig = (int)B.bigp.size();
+    if (nbig) {
+        k_btask<<<(unsigned)((nbig + 63) / 64), 64, 0, s>>>(v, B.bigp_d.p, nbig, B.dyn.p);
+        PE_LAUNCH_CHECK();
+    }
+    if (B.ncolblk) {
+        k_bmf_d<double><<<(unsigned)B.ncolblk, 256, 0, s>>>(v);
+        PE_LAUNCH_CHECK();
+        k_bmf_gemm<double><<<(unsigned)B.ntile, 256, 0, s>>>(v);
+        PE_LAUNCH_CHECK();
+        k_bmf_sigma<double><<<(unsigned)B.ncolblk, 256, 0, s>>>(v);
+        PE_LAUNCH_CHECK();
+    }
+    for (int i = 0; i < nbig; ++i) {
+        const BProb &P = B.hp[(size_t)B.bigp[(size_t)i]];
+        mf_sigma_batched<double>(B.bigv[(size_t)i], P.nd, B.kb0, B.V.p, B.W.p, (double *)B.fws.p, B.V64p.p + P.vint,
+                                 B.k64p.p + P.kv, B.dyn.p + 6 * i, P.sector, s);
+    }
     k_bcvt<<<(unsigned)B.nrowblk, 256, 0, s>>>(v, 0);
     PE_LAUNCH_CHECK();
-    k_bmf_d<float><<<(unsigned)B.ncolblk, 256, 0, s>>>(v);
-    PE_LAUNCH_CHECK();
-    k_bmf_gemm<float><<<(unsigned)B.ntile, 256, 0, s>>>(v);
-    PE_LAUNCH_CHECK();
-    k_bmf_sigma<float><<<(unsigned)B.ncolblk, 256, 0, s>>>(v);
-    PE_LAUNCH_CHECK();
+    if (B.ncolblk) {
+        k_bmf_d<float><<<(unsigned)B.ncolblk, 256, 0, s>>>(v);
+        PE_LAUNCH_CHECK();
+        k_bmf_gemm<float><<<(unsigned)B.ntile, 256, 0, s>>>(v);
+        PE_LAUNCH_CHECK();
+        k_bmf_sigma<float><<<(unsigned)B.ncolblk, 256, 0, s>>>(v);
+        PE_LAUNCH_CHECK();
+    }
+    for (int i = 0; i < nbig; ++i) {
+        const BProb &P = B.hp[(size_t)B.bigp[(size_t)i]];
+        mf_sigma_batched<float>(B.bigv[(size_t)i], P.nd, B.kb0, B.S32x.p, B.S32y.p, (float *)B.fws.p,
+                                B.V32p.p + P.vint, B.k32p.p + P.kv, B.dyn.p + 6 * i + 3, P.sector, s);
+    }
     k_bcvt<<<(unsigned)B.nrowblk, 256, 0, s>>>(v, 1);
     PE_LAUNCH_CHECK();
     // Rayleigh-matrix entries of the new columns
@@ -964,17 +1015,25 @@ void batch_solve(pyci_ctx *ctx, i64 nprob, const i64 *nbasis, const i64 *nup, co
             P.s32 = s32;
             s32 += (i64)B.kb0 * P.nd;
             const i64 cols = (i64)B.kb0 * P.nd;  // widest sigma: the start block
-            P.ldw = (cols + 127) / 128 * 128;
-            P.work = work;
-            work += (i64)P.kpad * P.ldw;
+            P.big = P.nd >= kBigND ? 1 : 0;
             const int sp = (int)B.hp.size();
-            for (i64 c = 0; c < cols; c += 256) {
-                col_blk.push_back(sp);
-                col_first.push_back(c);
-            }
-            for (i64 c = 0; c < cols; c += 128) {
-                tile_blk.push_back(sp);
-                tile_first.push_back(c);
+            if (P.big) {
+                // the shared workspace of the fused pass instead
+                P.ldw = 0;
+                P.work = 0;
+                B.bigp.push_back(sp);
+            } else {
+                P.ldw = (cols + 127) / 128 * 128;
+                P.work = work;
+                work += (i64)P.kpad * P.ldw;
+                for (i64 c = 0; c < cols; c += 256) {
+                    col_blk.push_back(sp);
+                    col_first.push_back(c);
+                }
+                for (i64 c = 0; c < cols; c += 128) {
+                    tile_blk.push_back(sp);
+                    tile_first.push_back(c);
+                }
             }
             P.ecore = ecore[p];
             P.swap_nb = (sector != 0) ? O.NB : 0;
@@ -998,6 +1057,34 @@ void batch_solve(pyci_ctx *ctx, i64 nprob, const i64 *nbasis, const i64 *nup, co
     PE_HIP(hipMemcpyAsync(B.k64p.p, k64.data(), sizeof(double) * k64.size(), hipMemcpyHostToDevice, s));
     PE_HIP(hipMemcpyAsync(B.V32p.p, V32.data(), sizeof(float) * V32.size(), hipMemcpyHostToDevice, s));
     PE_HIP(hipMemcpyAsync(B.k32p.p, k32.data(), sizeof(float) * k32.size(), hipMemcpyHostToDevice, s));
+    {
+        // fused-pass data of the large sub-problems
+        std::vector<int2> mt;
+        std::vector<i64> moff;
+        size_t ws = 0;
+        for (int q : B.bigp) {
+            const BProb &P = B.hp[(size_t)q];
+            const i64 na = P.nd / P.NB;
+            moff.push_back((i64)mt.size());
+            mt.insert(mt.end(), tabs.begin() + P.tab_a, tabs.begin() + P.tab_a + na * P.nea);
+            mt.insert(mt.end(), tabs.begin() + P.tab_b, tabs.begin() + P.tab_b + P.NB * P.neb);
+            mt.push_back(int2{0, mf_sink_row(P.kpad)});
+            ws = std::max(ws, (size_t)(((P.nd * B.kb0 + 63) / 64) * 64) * (size_t)P.kpad * sizeof(double));
+        }
+        if (!B.bigp.empty()) {
+            B.mtab.alloc(mt.size());
+            PE_HIP(hipMemcpyAsync(B.mtab.p, mt.data(), sizeof(int2) * mt.size(), hipMemcpyHostToDevice, s));
+            B.fws.alloc(ws);
+            B.dyn.alloc(6 * B.bigp.size());
+            B.bigp_d.alloc(B.bigp.size());
+            PE_HIP(hipMemcpyAsync(B.bigp_d.p, B.bigp.data(), sizeof(int) * B.bigp.size(), hipMemcpyHostToDevice, s));
+            for (size_t i = 0; i < B.bigp.size(); ++i) {
+                const BProb &P = B.hp[(size_t)B.bigp[i]];
+                B.bigv.push_back(MfView{B.mtab.p + moff[i], B.tab.p + P.tab_a, B.tab.p + P.tab_b, P.nd / P.NB, P.NB,
+                                        P.npair, P.kpad, P.nea, P.neb});
+            }
+        }
+    }
     B.diag.alloc((size_t)vec);
     B.V.alloc((size_t)basis);
     B.W.alloc((size_t)basis);

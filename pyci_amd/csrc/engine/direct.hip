@@ -121,18 +121,30 @@ __host__ __device__ constexpr int dprod_vc(int MT) {
     return fixed_npair(MT) ? 4 * ((fixed_npair(MT) + 3) / 4) : 16 * MT;
 }
 
+int mf_sink_row(int kpad) { return dprod_vc(kpad / 16); }
+
 struct DprodArgs {
     int ncol, NB, nvec, npair, nea, ntot;
     int vstride, v0;  // X[J][v] of vector v0 + v (v < nvec) at X[J * vstride + v0 + v]
+    int colmajor;     // or, with colmajor, at X[v * nd + J] (vectors contiguous)
+    int nd;
+    const long long *dyn;  // batched solver: device {vectors (0: nothing), X offset, Y offset}, or null
     int offb;  // first beta row of the merged table, NA * nea
     int sent;  // index of the sentinel entry
     int ld;
 };
 
 template <int MT, typename T, bool SB, int VM, int PE>
-__global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__restrict__ tab, const T *__restrict__ X,
+__global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__restrict__ tab, const T *__restrict__ X0,
                                                   const T *__restrict__ V, const T *__restrict__ kv,
                                                   T *__restrict__ W) {
+    const T *__restrict__ X = X0;
+    if (A.dyn) {
+        A.nvec = (int)A.dyn[0];
+        A.ncol = A.nd * A.nvec;
+        X += A.dyn[1];
+        if (A.ncol == 0) return;  // every block: nothing to multiply this step
+    }
     constexpr int KP = MT * 16, NT = kDprodThreads;
     constexpr int NP = fixed_npair(MT);             // 0: pair count known only at run time
     constexpr int VC = dprod_vc(MT), NQ = VC / 4;   // k extent and k steps
@@ -154,9 +166,9 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
     // The two roles run separate loops with the same barrier count, so that neither
     // role's registers are live in the other.
     if (wave >= 4) {
-        const int g = wave - 4, sa = A.NB * A.vstride;
+        const int g = wave - 4, sa = A.colmajor ? A.NB : A.NB * A.vstride, sb = A.colmajor ? 1 : A.vstride;
         // loads the table entries of the tile of `step` into nx and its X bases: the X
-        // index of an alpha entry is en.x * sa + na, of a beta entry en.x * vstride + nb
+        // index of an alpha entry is en.x * sa + na, of a beta entry en.x * sb + nb
         auto fetch = [&](int step, int2 *nx, int &na, int &nb) {
             const int c0 = ((int)blockIdx.x + step * (int)gridDim.x) * 64 + lane;
             const bool live = step < nmine && c0 < A.ncol;
@@ -164,8 +176,13 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
             const unsigned J = c / (unsigned)A.nvec, v = c - J * (unsigned)A.nvec;
             const unsigned Ja = J / (unsigned)A.NB, Jb = J - Ja * (unsigned)A.NB;
             const int ra = (int)(Ja * (unsigned)A.nea), rb = A.offb + (int)(Jb * (unsigned)(A.ntot - A.nea)) - A.nea;
-            na = (int)(Jb * (unsigned)A.vstride + v) + A.v0;
-            nb = (int)(Ja * (unsigned)sa + v) + A.v0;
+            if (A.colmajor) {
+                na = (int)(v * (unsigned)A.nd + Jb);
+                nb = (int)(v * (unsigned)A.nd + Ja * (unsigned)A.NB);
+            } else {
+                na = (int)(Jb * (unsigned)A.vstride + v) + A.v0;
+                nb = (int)(Ja * (unsigned)sa + v) + A.v0;
+            }
 #pragma unroll
             for (int u = 0; u < PE; ++u) {
                 const int t = g + 8 * u;
@@ -177,7 +194,7 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
 #pragma unroll
             for (int u = 0; u < PE; ++u) {
                 const int t = g + 8 * u;
-                val[u] = X[(t < A.nea) ? en[u].x * sa + xa : en[u].x * A.vstride + xb];
+                val[u] = X[(t < A.nea) ? en[u].x * sa + xa : en[u].x * sb + xb];
             }
         };
         auto add = [&](const int2 *en, T *val, double *buf) {
@@ -230,8 +247,8 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
         };
         // X of column c (determinant c / nvec, vector c % nvec), loaded one tile ahead
         auto xat = [&](int c) -> T {
-            const int cc = min(c, A.ncol - 1), J = cc / A.nvec;
-            return X[J * A.vstride + A.v0 + (cc - J * A.nvec)];
+            const int cc = min(c, A.ncol - 1), J = cc / A.nvec, v = cc - J * A.nvec;
+            return A.colmajor ? X[v * A.nd + J] : X[J * A.vstride + A.v0 + v];
         };
         T x = xat((int)blockIdx.x * 64 + 16 * wave + lr);
         auto product = [&](int tile, double *buf) {
@@ -311,8 +328,13 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
 template <int NV, typename T, bool TILED>
 __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int neb, int KP,
                                                   const int2 *__restrict__ ta, const int2 *__restrict__ tb, int nvec,
-                                                  int vstride, int v0, int alpha_only, const T *__restrict__ W,
-                                                  T *__restrict__ Y) {
+                                                  int vstride, int v0, int alpha_only, int colmajor,
+                                                  const long long *dyn, const T *__restrict__ W, T *__restrict__ Y) {
+    if (dyn) {
+        nvec = (int)dyn[0];
+        Y += dyn[2];
+        if (nvec == 0) return;
+    }
     i64 Ia, Ib;
     if (!det_of<TILED>(nd, nd / NB, NB, Ia, Ib)) return;
     const i64 I = Ia * NB + Ib;
@@ -337,10 +359,9 @@ __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int n
         for (int v = 0; v < NV; ++v)
             if (v < nvec) acc[v] += s * W[wt_at(c + v, e.y & 0xffff, KP)];
     }
-    T *y = Y + I * vstride + v0;
 #pragma unroll
     for (int v = 0; v < NV; ++v)
-        if (v < nvec) y[v] = acc[v];
+        if (v < nvec) Y[colmajor ? v * nd + I : I * vstride + v0 + v] = acc[v];
 }
 
 // Sigma of a vector in a parity sector of the alpha <-> beta exchange P (nup == ndn,
@@ -350,19 +371,25 @@ __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int n
 //   y(I) = A(I) + s A(P I),   A = the alpha terms (k_mf_sigma with alpha_only).
 // In place on Y: the thread of I with Ia <= Ib owns the pair (I, P I).
 template <typename T>
-__global__ void k_mf_symm(T *Y, i64 NB, int nvec, int vstride, int v0, T s) {
+__global__ void k_mf_symm(T *Y, i64 NB, int nvec, int vstride, int v0, int colmajor, const long long *dyn, T s) {
+    if (dyn) {
+        nvec = (int)dyn[0];
+        Y += dyn[2];
+        if (nvec == 0) return;
+    }
     const i64 nd = NB * NB;
+    const i64 sj = colmajor ? 1 : vstride, sv = colmajor ? nd : 1, o = colmajor ? 0 : v0;
     for (i64 I = (i64)blockIdx.x * blockDim.x + threadIdx.x; I < nd; I += (i64)gridDim.x * blockDim.x) {
         const i64 Ia = I / NB, Ib = I - Ia * NB;
         if (Ia > Ib) continue;
-        T *y0 = Y + I * vstride + v0, *y1 = Y + (Ib * NB + Ia) * vstride + v0;
+        T *y0 = Y + I * sj + o, *y1 = Y + (Ib * NB + Ia) * sj + o;
         for (int v = 0; v < nvec; ++v) {
-            const T a = y0[v], b = y1[v];
+            const T a = y0[v * sv], b = y1[v * sv];
             if (Ia == Ib) {
-                y0[v] = a + s * a;
+                y0[v * sv] = a + s * a;
             } else {
-                y0[v] = a + s * b;
-                y1[v] = b + s * a;
+                y0[v * sv] = a + s * b;
+                y1[v * sv] = b + s * a;
             }
         }
     }
@@ -611,8 +638,8 @@ static int device_attr(hipDeviceAttribute_t a) {
 static size_t lds_per_block() { return (size_t)device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock); }
 
 template <int MT, typename T, bool SB, int VM, int PE>
-static void launch_dprod_v(const DirectOp *D, const DprodArgs &A, const T *X, const T *V, const T *kv, T *W,
-                           size_t lds, hipStream_t s) {
+static void launch_dprod_v(const MfView &M, const DprodArgs &A, const T *X, const T *V, const T *kv, T *W, size_t lds,
+                           hipStream_t s) {
     auto kern = k_mf_dprod<MT, T, SB, VM, PE>;
     static bool attr = false;  // dynamic LDS above 64 KB
     if (!attr) {
@@ -626,7 +653,7 @@ static void launch_dprod_v(const DirectOp *D, const DprodArgs &A, const T *X, co
     static const i64 ncu = device_attr(hipDeviceAttributeMultiprocessorCount);
     const i64 ntiles = ((i64)A.ncol + 63) / 64;
     const unsigned grid = (unsigned)std::max<i64>(1, std::min<i64>(ntiles, ncu * std::max(1, per_cu)));
-    kern<<<grid, kDprodThreads, lds, s>>>(A, D->tab.p, X, V, kv, W);
+    kern<<<grid, kDprodThreads, lds, s>>>(A, M.tab, X, V, kv, W);
     PE_LAUNCH_CHECK();
 }
 
@@ -647,65 +674,89 @@ struct DprodPlan {
     static_assert(lds <= cap, "no LDS plan fits");
 };
 
+// How one call's vectors are laid out: C.nvec vectors (with C.dyn: at most that
+// many, the count and offsets read on the device), row-major X[J * vstride + v0 + v]
+// or column-major X[v * nd + J].
+struct MfCall {
+    i64 nd;
+    int nvec, vstride, v0, colmajor;
+    const long long *dyn;
+};
+
 template <int MT, typename T>
-static void launch_dprod(const DirectOp *D, i64 nd, int nvec, int vstride, int v0, const T *X, const T *V,
-                         const T *kv, T *W, hipStream_t s) {
+static void launch_dprod(const MfView &M, const MfCall &C, const T *X, const T *V, const T *kv, T *W, hipStream_t s) {
     typedef DprodPlan<MT, T> P;
     static const size_t cap = lds_per_block();
-    const int ntot = D->nea + D->neb;
+    const int ntot = M.nea + M.neb;
     PE_REQUIRE(ntot <= 128, PYCI_EUNSUPPORTED, "too many single excitations per determinant");
-    PE_REQUIRE(nd * nvec < ((i64)1 << 31) - ((i64)1 << 20), PYCI_EUNSUPPORTED,
-               "too many columns for the matrix-free operator");
-    PE_REQUIRE(!fixed_npair(MT) || D->npair == fixed_npair(MT), PYCI_EUNSUPPORTED,
+    PE_REQUIRE(C.nd * C.nvec < ((i64)1 << 31) - ((i64)1 << 20) && C.nd * C.vstride < ((i64)1 << 31),
+               PYCI_EUNSUPPORTED, "too many columns for the matrix-free operator");
+    PE_REQUIRE(!fixed_npair(MT) || M.npair == fixed_npair(MT), PYCI_EUNSUPPORTED,
                "pair count does not match the pair dimension");
     PE_REQUIRE(P::lds <= cap, PYCI_EUNSUPPORTED, "the device has too little LDS for the matrix-free product");
-    PE_REQUIRE(nd * vstride < ((i64)1 << 31), PYCI_EUNSUPPORTED, "too many columns for the matrix-free operator");
-    DprodArgs A{(int)(nd * nvec), (int)D->nbs, nvec, D->npair, D->nea, ntot, vstride, v0, (int)(D->na * D->nea),
-                (int)(D->na * D->nea + D->nbs * D->neb), P::ld};
-    if (ntot <= 64) launch_dprod_v<MT, T, P::SB, P::VM, 8>(D, A, X, V, kv, W, P::lds, s);
-    else if (ntot <= 96) launch_dprod_v<MT, T, P::SB, P::VM, 12>(D, A, X, V, kv, W, P::lds, s);
-    else launch_dprod_v<MT, T, P::SB, P::VM, 16>(D, A, X, V, kv, W, P::lds, s);
+    DprodArgs A{(int)(C.nd * C.nvec), (int)M.nbs, C.nvec, M.npair, M.nea, ntot, C.vstride, C.v0, C.colmajor,
+                (int)C.nd, C.dyn, (int)(M.na * M.nea), (int)(M.na * M.nea + M.nbs * M.neb), P::ld};
+    if (ntot <= 64) launch_dprod_v<MT, T, P::SB, P::VM, 8>(M, A, X, V, kv, W, P::lds, s);
+    else if (ntot <= 96) launch_dprod_v<MT, T, P::SB, P::VM, 12>(M, A, X, V, kv, W, P::lds, s);
+    else launch_dprod_v<MT, T, P::SB, P::VM, 16>(M, A, X, V, kv, W, P::lds, s);
 }
 
 // ev (optional): 3 events recorded before, between and after the two passes.
 template <int NV, typename T>
-static void launch_passes(const DirectOp *D, i64 nd, int nvec, int vstride, int v0, const T *X, T *W, T *Y,
-                          const T *V, const T *kv, hipStream_t s, hipEvent_t *ev, int sector) {
+static void launch_passes(const MfView &M, const MfCall &C, const T *X, T *W, T *Y, const T *V, const T *kv,
+                          hipStream_t s, hipEvent_t *ev, int sector) {
     if (ev) PE_HIP(hipEventRecord(ev[0], s));
-    switch (D->kpad / 16) {
-        case 1: launch_dprod<1, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
-        case 2: launch_dprod<2, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
-        case 3: launch_dprod<3, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
-        case 4: launch_dprod<4, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
-        case 5: launch_dprod<5, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
-        case 6: launch_dprod<6, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
-        case 7: launch_dprod<7, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
-        case 8: launch_dprod<8, T>(D, nd, nvec, vstride, v0, X, V, kv, W, s); break;
+    switch (M.kpad / 16) {
+        case 1: launch_dprod<1, T>(M, C, X, V, kv, W, s); break;
+        case 2: launch_dprod<2, T>(M, C, X, V, kv, W, s); break;
+        case 3: launch_dprod<3, T>(M, C, X, V, kv, W, s); break;
+        case 4: launch_dprod<4, T>(M, C, X, V, kv, W, s); break;
+        case 5: launch_dprod<5, T>(M, C, X, V, kv, W, s); break;
+        case 6: launch_dprod<6, T>(M, C, X, V, kv, W, s); break;
+        case 7: launch_dprod<7, T>(M, C, X, V, kv, W, s); break;
+        case 8: launch_dprod<8, T>(M, C, X, V, kv, W, s); break;
         default: throw EngineError(PYCI_EUNSUPPORTED, "pair dimension too large for the matrix-free operator");
     }
     if (ev) PE_HIP(hipEventRecord(ev[1], s));
+    const i64 nd = C.nd;
     if (sector != 0) {
         // alpha terms (every block of 256 consecutive determinants shares few alpha
         // strings, so their table rows are common and the E reads contiguous), then
         // the parity symmetrisation
-        k_mf_sigma<NV, T, false><<<grid_for(nd, 256), 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->kpad, D->tab_a.p,
-                                                                   D->tab_b.p, nvec, vstride, v0, 1, W, Y);
+        k_mf_sigma<NV, T, false><<<grid_for(nd, 256), 256, 0, s>>>(nd, M.nbs, M.nea, M.neb, M.kpad, M.tab_a, M.tab_b,
+                                                                   C.nvec, C.vstride, C.v0, 1, C.colmajor, C.dyn, W,
+                                                                   Y);
         PE_LAUNCH_CHECK();
-        k_mf_symm<T><<<grid_for(nd, 256), 256, 0, s>>>(Y, D->nbs, nvec, vstride, v0, (T)sector);
+        k_mf_symm<T><<<grid_for(nd, 256), 256, 0, s>>>(Y, M.nbs, C.nvec, C.vstride, C.v0, C.colmajor, C.dyn,
+                                                       (T)sector);
         PE_LAUNCH_CHECK();
         if (ev) PE_HIP(hipEventRecord(ev[2], s));
         return;
     }
-    const bool tiled = (D->nbs + 15) / 16 >= 16;  // enough column tiles to spread over the XCDs
-    const unsigned g = tiled ? tiled_grid(D->na, D->nbs) : grid_for(nd, 256);
+    const bool tiled = (M.nbs + 15) / 16 >= 16;  // enough column tiles to spread over the XCDs
+    const unsigned g = tiled ? tiled_grid(M.na, M.nbs) : grid_for(nd, 256);
     if (tiled)
-        k_mf_sigma<NV, T, true><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->kpad, D->tab_a.p, D->tab_b.p, nvec,
-                                                  vstride, v0, 0, W, Y);
+        k_mf_sigma<NV, T, true><<<g, 256, 0, s>>>(nd, M.nbs, M.nea, M.neb, M.kpad, M.tab_a, M.tab_b, C.nvec,
+                                                  C.vstride, C.v0, 0, C.colmajor, C.dyn, W, Y);
     else
-        k_mf_sigma<NV, T, false><<<g, 256, 0, s>>>(nd, D->nbs, D->nea, D->neb, D->kpad, D->tab_a.p, D->tab_b.p,
-                                                   nvec, vstride, v0, 0, W, Y);
+        k_mf_sigma<NV, T, false><<<g, 256, 0, s>>>(nd, M.nbs, M.nea, M.neb, M.kpad, M.tab_a, M.tab_b, C.nvec,
+                                                   C.vstride, C.v0, 0, C.colmajor, C.dyn, W, Y);
     PE_LAUNCH_CHECK();
     if (ev) PE_HIP(hipEventRecord(ev[2], s));
+}
+
+template <typename T>
+static void launch_passes_nv(const MfView &M, const MfCall &C, const T *X, T *W, T *Y, const T *V, const T *kv,
+                             hipStream_t s, hipEvent_t *ev, int sector) {
+    if (C.nvec == 1) launch_passes<1, T>(M, C, X, W, Y, V, kv, s, ev, sector);
+    else if (C.nvec == 2) launch_passes<2, T>(M, C, X, W, Y, V, kv, s, ev, sector);
+    else if (C.nvec <= 4) launch_passes<4, T>(M, C, X, W, Y, V, kv, s, ev, sector);
+    else if (C.nvec <= 8) launch_passes<8, T>(M, C, X, W, Y, V, kv, s, ev, sector);
+    else launch_passes<16, T>(M, C, X, W, Y, V, kv, s, ev, sector);
+}
+
+static MfView view_of(const DirectOp *D) {
+    return MfView{D->tab.p, D->tab_a.p, D->tab_b.p, D->na, D->nbs, D->npair, D->kpad, D->nea, D->neb};
 }
 
 // The vectors go through the two passes in chunks whose E workspace stays within
@@ -724,16 +775,28 @@ void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, 
     T *W = direct_workspace<T>(op, ((nd * chunk + 63) / 64 * 64) * (i64)D->kpad);
     const T *V = std::is_same<T, double>::value ? (const T *)D->V64.p : (const T *)D->V32.p;
     const T *kv = std::is_same<T, double>::value ? (const T *)D->k64.p : (const T *)D->k32.p;
+    const MfView M = view_of(D);
     for (int v0 = 0; v0 < nvec; v0 += chunk) {
         const int nc = std::min(chunk, nvec - v0);
         hipEvent_t *e = (ev && v0 == 0 && nc == nvec) ? ev : nullptr;
-        if (nc == 1) launch_passes<1, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e, sector);
-        else if (nc == 2) launch_passes<2, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e, sector);
-        else if (nc <= 4) launch_passes<4, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e, sector);
-        else if (nc <= 8) launch_passes<8, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e, sector);
-        else launch_passes<16, T>(D, nd, nc, nvec, v0, X, W, Y, V, kv, s, e, sector);
+        launch_passes_nv<T>(M, MfCall{nd, nc, nvec, v0, 0, nullptr}, X, W, Y, V, kv, s, e, sector);
     }
 }
+
+// Sigma of a batched-solver problem (batch.hip): up to nvec_max column-major
+// vectors, their count and the offsets of X and Y read on the device from dyn at
+// run time (a launch inside a captured graph that finds a count of 0 returns at
+// once). W holds kpad x (nd nvec_max) values.
+template <typename T>
+void mf_sigma_batched(const MfView &M, i64 nd, int nvec_max, const T *X, T *Y, T *W, const T *V, const T *kv,
+                      const long long *dyn, int sector, hipStream_t s) {
+    launch_passes_nv<T>(M, MfCall{nd, nvec_max, 1, 0, 1, dyn}, X, W, Y, V, kv, s, nullptr, sector);
+}
+
+template void mf_sigma_batched<double>(const MfView &, i64, int, const double *, double *, double *, const double *,
+                                       const double *, const long long *, int, hipStream_t);
+template void mf_sigma_batched<float>(const MfView &, i64, int, const float *, float *, float *, const float *,
+                                      const float *, const long long *, int, hipStream_t);
 
 template void direct_sigma<double>(const pyci_op *, int, const double *, double *, hipStream_t, hipEvent_t *, int);
 template void direct_sigma<float>(const pyci_op *, int, const float *, float *, hipStream_t, hipEvent_t *, int);

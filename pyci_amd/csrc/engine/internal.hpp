@@ -342,6 +342,18 @@ struct HciPool {
 void hci_pool_device(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double eps, double eps_pool, HciPool &P);
 
 // Matrix-free FullCI operator (direct.hip)
+// What the sigma passes read of a matrix-free operator: the alpha and beta string
+// tables, the merged table with its sentinel, counts.
+struct MfView {
+    const int2 *tab, *tab_a, *tab_b;
+    i64 na, nbs;
+    int npair, kpad, nea, neb;
+};
+template <typename T>
+void mf_sigma_batched(const MfView &M, i64 nd, int nvec_max, const T *X, T *Y, T *W, const T *V, const T *kv,
+                      const long long *dyn, int sector, hipStream_t s);
+// pair dimension kpad -> k extent of the fused pass (its sink row index)
+int mf_sink_row(int kpad);
 bool direct_applicable(const pyci_ham *ham, const pyci_wfn *wfn, i64 nrow, i64 ncol, int symmetric);
 void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn);
 template <typename T>
