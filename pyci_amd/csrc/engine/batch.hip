@@ -1067,7 +1067,8 @@ void batch_solve(pyci_ctx *ctx, i64 nprob, const i64 *nbasis, const i64 *nup, co
             const i64 na = P.nd / P.NB;
             moff.push_back((i64)mt.size());
             mt.insert(mt.end(), tabs.begin() + P.tab_a, tabs.begin() + P.tab_a + na * P.nea);
-            mt.insert(mt.end(), tabs.begin() + P.tab_b, tabs.begin() + P.tab_b + P.NB * P.neb);
+            for (int t = 0; t < P.neb; ++t)  // beta entries slot-major
+                for (i64 r = 0; r < P.NB; ++r) mt.push_back(tabs[(size_t)(P.tab_b + r * P.neb + t)]);
             mt.push_back(int2{0, mf_sink_row(P.kpad)});
             ws = std::max(ws, (size_t)(((P.nd * B.kb0 + 63) / 64) * 64) * (size_t)P.kpad * sizeof(double));
         }

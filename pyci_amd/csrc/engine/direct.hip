@@ -91,9 +91,9 @@ __host__ __device__ inline i64 wt_at(i64 c, int p, int KP) { return (c >> 6) * (
 // gather waves add tile t into the buffer between two barriers and load the X
 // values of tile t+1 into registers while the product of tile t runs.
 // Gather: lane = column; wave 4 + g takes table entries g, g + 8, ... (PE slots,
-// 8 PE >= nea + neb) of the merged table (alpha rows, then beta rows, then one
-// sentinel entry that every empty slot reads: it adds into the last buffer row,
-// which nothing reads), loading the entries of its next tile while X values are in
+// 8 PE >= nea + neb) of the merged table (alpha rows, then the beta entries slot by
+// slot, then one sentinel entry that every empty slot reads: it adds into the last
+// buffer row, which nothing reads), loading the entries of its next tile while X values are in
 // flight. Indices are 32-bit (nd nvec < 2^31). A (pair, column) cell receives at
 // most one alpha and one beta term (a string's entries have distinct pairs), added
 // onto zero, so the LDS atomic adds give the same bits in any order. D is held in
@@ -129,7 +129,7 @@ struct DprodArgs {
     int colmajor;     // or, with colmajor, at X[v * nd + J] (vectors contiguous)
     int nd;
     const long long *dyn;  // batched solver: device {vectors (0: nothing), X offset, Y offset}, or null
-    int offb;  // first beta row of the merged table, NA * nea
+    int offb;  // start of the beta entries in the merged table, NA * nea (entry t of beta string r at offb + t NB + r)
     int sent;  // index of the sentinel entry
     int ld;
 };
@@ -175,7 +175,9 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
             const unsigned c = live ? (unsigned)c0 : 0u;
             const unsigned J = c / (unsigned)A.nvec, v = c - J * (unsigned)A.nvec;
             const unsigned Ja = J / (unsigned)A.NB, Jb = J - Ja * (unsigned)A.NB;
-            const int ra = (int)(Ja * (unsigned)A.nea), rb = A.offb + (int)(Jb * (unsigned)(A.ntot - A.nea)) - A.nea;
+            // alpha rows string-major; beta entries slot-major (entry t of every beta
+            // string contiguous, so the lanes' loads of one slot are one run)
+            const int ra = (int)(Ja * (unsigned)A.nea), rb = A.offb + (int)Jb - A.nea * A.NB;
             if (A.colmajor) {
                 na = (int)(v * (unsigned)A.nd + Jb);
                 nb = (int)(v * (unsigned)A.nd + Ja * (unsigned)A.NB);
@@ -186,7 +188,7 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
 #pragma unroll
             for (int u = 0; u < PE; ++u) {
                 const int t = g + 8 * u;
-                const int at = (t < A.nea) ? ra + t : rb + t;
+                const int at = (t < A.nea) ? ra + t : rb + t * A.NB;
                 nx[u] = tab[(live && t < A.ntot) ? at : A.sent];
             }
         };
@@ -341,24 +343,33 @@ __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int n
     T acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = (T)0;
-    const int2 *pa = ta + Ia * nea;
-    for (int t = 0; t < nea; ++t) {
-        const int2 e = pa[t];
-        const T s = (e.y >> 16) ? (T)-1 : (T)1;
-        const i64 c = ((i64)e.x * NB + Ib) * nvec;
+    // CH table entries, then their CH x NV E values, are loaded before any is used:
+    // the loads of a chunk are independent, so a wave waits twice per chunk instead
+    // of twice per entry. Indices past the row and vectors past nvec are clamped to
+    // valid ones and their values dropped.
+    constexpr int CH = NV >= 8 ? 2 : 16 / NV;
+    const auto terms = [&](const int2 *row, int n, bool alpha) {
+        for (int t0 = 0; t0 < n; t0 += CH) {
+            int2 e[CH];
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
-            if (v < nvec) acc[v] += s * W[wt_at(c + v, e.y & 0xffff, KP)];
-    }
-    const int2 *pb = tb + Ib * neb;
-    for (int t = 0; t < (alpha_only ? 0 : neb); ++t) {
-        const int2 e = pb[t];
-        const T s = (e.y >> 16) ? (T)-1 : (T)1;
-        const i64 c = (Ia * NB + e.x) * nvec;
+            for (int u = 0; u < CH; ++u) e[u] = row[min(t0 + u, n - 1)];
+            T w[CH][NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
-            if (v < nvec) acc[v] += s * W[wt_at(c + v, e.y & 0xffff, KP)];
-    }
+            for (int u = 0; u < CH; ++u) {
+                const i64 c = (alpha ? (i64)e[u].x * NB + Ib : Ia * NB + e[u].x) * nvec;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) w[u][v] = W[wt_at(c + min(v, nvec - 1), e[u].y & 0xffff, KP)];
+            }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const T sg = (t0 + u >= n) ? (T)0 : (e[u].y >> 16) ? (T)-1 : (T)1;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc[v] += sg * w[u][v];
+            }
+        }
+    };
+    terms(ta + Ia * nea, nea, true);
+    if (!alpha_only) terms(tb + Ib * neb, neb, false);
 #pragma unroll
     for (int v = 0; v < NV; ++v)
         if (v < nvec) Y[colmajor ? v * nd + I : I * vstride + v0 + v] = acc[v];
@@ -567,7 +578,8 @@ void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn) {
     D->tab_a.alloc(ta.size());
     D->tab_b.alloc(tb.size());
     std::vector<int2> merged(ta);
-    merged.insert(merged.end(), tb.begin(), tb.end());
+    for (int t = 0; t < D->neb; ++t)
+        for (i64 r = 0; r < (i64)sb.size(); ++r) merged.push_back(tb[(size_t)r * D->neb + t]);
     merged.push_back(int2{0, dprod_vc(D->kpad / 16)});  // sentinel: empty slots of the fused pass add into the sink row
     D->tab.alloc(merged.size());
     PE_HIP(hipMemcpyAsync(D->tab.p, merged.data(), sizeof(int2) * merged.size(), hipMemcpyHostToDevice, s));

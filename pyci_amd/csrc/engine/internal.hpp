@@ -163,7 +163,7 @@ struct DirectOp {
     double ecore = 0.0;
     bool csr_ready = false;      // the CSR arrays of the operator have been materialised
     DevBuf<int2> tab_a, tab_b;   // [string][entry] {rank of the excited string, pair | negative << 16}
-    DevBuf<int2> tab;            // tab_a rows, tab_b rows, then the sentinel {0, -1}
+    DevBuf<int2> tab;            // tab_a rows, tab_b entries slot-major (entry t of every beta string together), then the sentinel
     DevBuf<u64> str_a, str_b;    // strings in colex order
     DevBuf<double> one_mo, two_mo, V64, k64;
     DevBuf<float> V32, k32;
