@@ -52,15 +52,60 @@ __device__ inline void block_sum_to(double v, double *dst) {
     }
 }
 
+// out[q] = sum over b < npart of p[q][b] for N arrays of one value per block, all
+// threads of the block taking part (a fixed order: every block gets the same bits).
+template <int N>
+__device__ void sum_partials(const double *const (&p)[N], int npart, double *out) {
+    __shared__ double red[DT / 64][N];
+    double v[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = 0.0;
+    for (int b = threadIdx.x; b < npart; b += DT)
+#pragma unroll
+        for (int q = 0; q < N; ++q) v[q] += p[q][b];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o);
+        if (lane == 0) red[wave][q] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
+        double t = 0.0;
+        for (int w = 0; w < DT / 64; ++w) t += red[w][threadIdx.x];
+        out[threadIdx.x] = t;
+    }
+    __syncthreads();
+}
+
 // Second classical Gram-Schmidt pass only where the first removed more than half
 // of the squared norm (|t1|^2 < |t0|^2 / 2): "twice is enough" with the
 // Daniel-Gragg-Kaufman-Stewart test, decided on the device from the partial norms
 // of the copy (p0) and of the first pass (p1).
 __device__ bool second_pass(const double *p0, const double *p1, int npart) {
     __shared__ double nn[2];
-    reduce_partials_block(p0, npart, 1, nn);
-    reduce_partials_block(p1, npart, 1, nn + 1);
+    const double *const pp[2] = {p0, p1};
+    sum_partials<2>(pp, npart, nn);
     return nn[1] < 0.5 * nn[0];
+}
+
+// Sums of acc[i] (i < M) over the block to out[i].
+template <int M>
+__device__ void block_sums_to(double (&acc)[M], double *out) {
+    __shared__ double red[DT / 64][M];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        double v = acc[i];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) red[wave][i] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < M) {
+        double v = 0.0;
+        for (int w = 0; w < DT / 64; ++w) v += red[w][threadIdx.x];
+        out[threadIdx.x] = v;
+    }
 }
 
 // partial[b][ma][mb] = sum over rows of block b of A[:, i] * B[:, j] (8x8 tile per
@@ -172,6 +217,161 @@ __global__ void __launch_bounds__(DT) k_update_norm(i64 nd, const double *V, int
     block_sum_to(ss, npart + blockIdx.x);
 }
 
+// Basis-size-specialised forms of the one-column passes (M = the number of basis
+// columns, 1..16): every thread handles two rows 256 apart per step and issues all
+// their loads before the arithmetic. Columns are nd apart.
+
+// partial[b * M + i] = sum over the rows of block b of A[:, i] x; gated like
+// k_gram. With hnb > 0 (nd = hnb^2, the columns of A and x in one parity sector of
+// the alpha <-> beta exchange P: z(P r) = +-z(r) with the same sign for all) only the
+// rows (Ia, Ib) = (r / hnb, r % hnb) with Ia <= Ib are read; a row with Ia < Ib also
+// stands for its partner (Ib, Ia) and counts twice.
+template <int M>
+__global__ void __launch_bounds__(DT) k_dots(i64 nd, const double *__restrict__ A, const double *__restrict__ x,
+                                             double *partial, const double *gate0, const double *gate1,
+                                             unsigned hnb) {
+    if (gate0 && !second_pass(gate0, gate1, gridDim.x)) return;
+    double acc[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) acc[i] = 0.0;
+    const i64 step = (i64)gridDim.x * 2 * DT;
+    for (i64 r0 = (i64)blockIdx.x * 2 * DT + threadIdx.x; r0 < nd; r0 += step) {
+        double wt[2], xr[2], a[2][M];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const i64 r = r0 + u * DT;
+            wt[u] = (r < nd) ? 1.0 : 0.0;
+            if (hnb && r < nd) {
+                const unsigned ia = (unsigned)r / hnb, ib = (unsigned)r - ia * hnb;
+                wt[u] = ia < ib ? 2.0 : ia == ib ? 1.0 : 0.0;
+            }
+            xr[u] = 0.0;
+#pragma unroll
+            for (int i = 0; i < M; ++i) a[u][i] = 0.0;
+            if (wt[u] != 0.0) {
+                xr[u] = x[r];
+#pragma unroll
+                for (int i = 0; i < M; ++i) a[u][i] = A[(i64)i * nd + r];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int i = 0; i < M; ++i) acc[i] += a[u][i] * (wt[u] * xr[u]);
+    }
+    block_sums_to<M>(acc, partial + (i64)blockIdx.x * M);
+}
+
+// t -= V[:, 0:M] c, c the sums of cpart (M per block), and the partial squared
+// norms of the result per block; gated like k_update_norm.
+template <int M>
+__global__ void __launch_bounds__(DT) k_update_norm_m(i64 nd, const double *__restrict__ V,
+                                                      const double *cpart, double *t, double *npart,
+                                                      const double *gate0, const double *gate1) {
+    if (gate0 && !second_pass(gate0, gate1, gridDim.x)) return;
+    __shared__ double c[M];
+    reduce_partials_block(cpart, gridDim.x, M, c);
+    double cr[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) cr[i] = c[i];
+    double ss = 0.0;
+    const i64 step = (i64)gridDim.x * 2 * DT;
+    for (i64 r0 = (i64)blockIdx.x * 2 * DT + threadIdx.x; r0 < nd; r0 += step) {
+        double a[2][M], tv[2];
+        i64 r[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            r[u] = min(r0 + u * DT, nd - 1);
+            tv[u] = t[r[u]];
+#pragma unroll
+            for (int i = 0; i < M; ++i) a[u][i] = V[(i64)i * nd + r[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < M; ++i) acc += a[u][i] * cr[i];
+            const double x = tv[u] - acc;
+            if (r0 + u * DT < nd) {
+                t[r[u]] = x;
+                ss += x * x;
+            }
+        }
+    }
+    block_sum_to(ss, npart + blockIdx.x);
+}
+
+// One root, the speculative iteration: with the Ritz coefficients y (M) and theta,
+// per row x = V y, wx = W y, res = wx - theta x and the correction
+// t = res / (theta - d) (d the diagonal, the denominator kept off zero as in
+// k_ritz), written to T. Per block: the squared residual norm (pres), the squared
+// norm of t (pnorm) and the first Gram-Schmidt coefficients V^T t (pdots, M per
+// block). With the basis, its images and d all in one parity sector (d symmetric
+// under P), t is in that sector to the last bit, with no projection.
+template <int M>
+__global__ void __launch_bounds__(DT) k_ritz1(i64 nd, const double *__restrict__ V, const double *__restrict__ W,
+                                              const double *Y, const double *theta, const double *__restrict__ diag,
+                                              double *__restrict__ T, double *pres, double *pnorm, double *pdots) {
+    double y[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) y[i] = Y[i];
+    const double th = theta[0];
+    double acc[M + 2];
+#pragma unroll
+    for (int i = 0; i < M + 2; ++i) acc[i] = 0.0;
+    const i64 step = (i64)gridDim.x * 2 * DT;
+    for (i64 r0 = (i64)blockIdx.x * 2 * DT + threadIdx.x; r0 < nd; r0 += step) {
+        double v[2][M], w[2][M], d[2];
+        i64 r[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            r[u] = min(r0 + u * DT, nd - 1);
+            d[u] = diag[r[u]];
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                v[u][i] = V[(i64)i * nd + r[u]];
+                w[u][i] = W[(i64)i * nd + r[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (r0 + u * DT >= nd) break;
+            double x = 0.0, wx = 0.0;
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                x += v[u][i] * y[i];
+                wx += w[u][i] * y[i];
+            }
+            const double res = wx - th * x;
+            double den = th - d[u];
+            if (fabs(den) < 1e-8) den = (den < 0.0) ? -1e-8 : 1e-8;
+            const double t = res / den;
+            T[r[u]] = t;
+            acc[M] += res * res;
+            acc[M + 1] += t * t;
+#pragma unroll
+            for (int i = 0; i < M; ++i) acc[i] += v[u][i] * t;
+        }
+    }
+    __shared__ double out[M + 2];
+    block_sums_to<M + 2>(acc, out);
+    __syncthreads();
+    if (threadIdx.x < M) pdots[(i64)blockIdx.x * M + threadIdx.x] = out[threadIdx.x];
+    if (threadIdx.x == 0) {
+        pres[blockIdx.x] = out[M];
+        pnorm[blockIdx.x] = out[M + 1];
+    }
+}
+
+// d[r] = (diag[r] + diag[P r]) / 2 for P's index map (r = Ia nb + Ib -> Ib nb + Ia):
+// the same bits at r and P r.
+__global__ void k_symdiag(const double *diag, double *d, i64 nd, i64 nb) {
+    for (i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x; r < nd; r += (i64)gridDim.x * blockDim.x) {
+        const i64 a = r / nb, b = r - a * nb;
+        d[r] = 0.5 * (diag[r] + diag[b * nb + a]);
+    }
+}
+
 // dst = src, or dst = (src + sgn P src) / 2 with P the alpha <-> beta exchange of
 // a full FullCI space in colex order with nup == ndn (index Ja * nb + Jb <->
 // Jb * nb + Ja) when sgn != 0; the partial squared norms of dst per block.
@@ -267,9 +467,8 @@ __global__ void k_scale_checked(double *x, i64 n, const double *nn) {
 __global__ void __launch_bounds__(DT) k_scale_final(double *x, i64 n, const double *p0, const double *p1,
                                                     const double *p2, int npart, int gated, double *pair) {
     __shared__ double nn[3];
-    reduce_partials_block(p0, npart, 1, nn);
-    reduce_partials_block(p1, npart, 1, nn + 1);
-    if (gated) reduce_partials_block(p2, npart, 1, nn + 2);
+    const double *const pp[3] = {p0, p1, gated ? p2 : p1};
+    sum_partials<3>(pp, npart, nn);
     const double nf = (gated && nn[1] < 0.5 * nn[0]) ? nn[2] : nn[1];
     const double s = checked_scale(nn[0], nf);
     for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) x[i] *= s;
@@ -293,10 +492,12 @@ __global__ void k_parity(const double *src, double *dst, i64 nd, i64 nb, double 
 __global__ void k_start(double *x, i64 nd, u64 seed, double amp, i64 nb, double sgn, i64 d0, double w0, i64 d1,
                         double w1) {
     for (i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x; r < nd; r += (i64)gridDim.x * blockDim.x) {
-        double v = amp * ((double)(mix64(seed + (u64)r) >> 11) * 0x1p-53 - 0.5);
+        // both products rounded on their own (no contraction into the sum), so that
+        // v(P r) = sgn v(r) to the last bit
+        double v = __dmul_rn(amp, (double)(mix64(seed + (u64)r) >> 11) * 0x1p-53 - 0.5);
         if (sgn != 0.0) {
             const i64 pr = (r % nb) * nb + r / nb;
-            v = 0.5 * (v + sgn * amp * ((double)(mix64(seed + (u64)pr) >> 11) * 0x1p-53 - 0.5));
+            v = 0.5 * (v + sgn * __dmul_rn(amp, (double)(mix64(seed + (u64)pr) >> 11) * 0x1p-53 - 0.5));
         }
         if (r == d0) v += w0;
         if (r == d1) v += w1;
@@ -388,6 +589,9 @@ struct Davidson {
     int nblk;
     int sector = 0;   // 0, or +1 / -1: the basis is kept in that parity sector of P
     i64 swap_nb = 0;  // beta-string count of P's index map
+    bool exact = true;          // sigma outputs in the basis' sector to the last bit
+    DevBuf<double> dsym;        // in a sector: the diagonal made symmetric under P
+    const double *dg = nullptr; // the diagonal used by the preconditioner
     pyci_solve_stats st{};
     // dv / hb layout (doubles): Ritz coefficients Y (m x k) and theta go down; the
     // squared residual norms, the new G block and the norm pairs of the added
@@ -431,6 +635,67 @@ struct Davidson {
         pC2 = pC1 + NB * kCgsMax;
         pb.alloc(pC2 + NB * kCgsMax);
         G.assign((size_t)mmax * mmax, 0.0);
+        dg = op->diag.p;
+    }
+
+    // Keeps the basis in parity sector sgn (0: none): the preconditioner then uses
+    // the diagonal symmetrised under P, so that corrections stay in the sector bit
+    // for bit (needs nd = nb^2).
+    void set_sector(int sgn, i64 nb) {
+        sector = sgn;
+        swap_nb = nb;
+        // the matrix-free sector sigma builds y(P I) from the same terms as y(I);
+        // the CSR product sums the two rows in different orders
+        exact = sgn == 0 || op->storage == PYCI_STORAGE_DIRECT;
+        if (sgn == 0) return;
+        PE_REQUIRE(nb > 0 && nb * nb == nd && nb < ((i64)1 << 31), PYCI_EINVAL, "parity sector needs nd = nb^2");
+        dsym.alloc((size_t)nd);
+        k_symdiag<<<g1(), 256, 0, s>>>(op->diag.p, dsym.p, nd, nb);
+        PE_LAUNCH_CHECK();
+        dg = dsym.p;
+    }
+    unsigned hnb() const { return sector != 0 ? (unsigned)swap_nb : 0u; }
+
+    // partial[b * m + i] = block b's part of A[:, i] . x, i < m <= 16 (k_dots).
+    void dots(const double *A, int m_, const double *x, double *part, const double *g0, const double *g1,
+              unsigned half) {
+        switch (m_) {
+#define PE_DOTS(M)                                                                                  \
+    case M:                                                                                         \
+        k_dots<M><<<(unsigned)nblk, DT, 0, s>>>(nd, A, x, part, g0, g1, half);                    \
+        break;
+            PE_DOTS(1) PE_DOTS(2) PE_DOTS(3) PE_DOTS(4) PE_DOTS(5) PE_DOTS(6) PE_DOTS(7) PE_DOTS(8)
+            PE_DOTS(9) PE_DOTS(10) PE_DOTS(11) PE_DOTS(12) PE_DOTS(13) PE_DOTS(14) PE_DOTS(15) PE_DOTS(16)
+#undef PE_DOTS
+            default: throw EngineError(PYCI_EUNSUPPORTED, "basis too large for the one-column passes");
+        }
+        PE_LAUNCH_CHECK();
+    }
+
+    // t -= V[:, 0:m] c (c from the partials cp), the partial squared norms to np; m <= 16.
+    void update(int m_, const double *cp, double *t, double *np, const double *g0, const double *g1) {
+        switch (m_) {
+#define PE_UPD(M)                                                                                   \
+    case M:                                                                                         \
+        k_update_norm_m<M><<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, cp, t, np, g0, g1);             \
+        break;
+            PE_UPD(1) PE_UPD(2) PE_UPD(3) PE_UPD(4) PE_UPD(5) PE_UPD(6) PE_UPD(7) PE_UPD(8)
+            PE_UPD(9) PE_UPD(10) PE_UPD(11) PE_UPD(12) PE_UPD(13) PE_UPD(14) PE_UPD(15) PE_UPD(16)
+#undef PE_UPD
+            default: throw EngineError(PYCI_EUNSUPPORTED, "basis too large for the one-column passes");
+        }
+        PE_LAUNCH_CHECK();
+    }
+
+    // dst (m values) = A[:, 0:m]^T x; half: A and x in the basis' parity sector
+    void dots_to(const double *A, int m_, const double *x, double *dst, bool half) {
+        if (m_ > 16) {
+            gram(A, m_, x, 1, dst);
+            return;
+        }
+        dots(A, m_, x, partial.p, nullptr, nullptr, half ? hnb() : 0u);
+        k_reduce_partials<<<(unsigned)m_, 256, 0, s>>>(partial.p, nblk, m_, dst);
+        PE_LAUNCH_CHECK();
     }
 
     unsigned g1() const { return (unsigned)std::min<i64>(4096, (nd + 255) / 256); }
@@ -512,13 +777,20 @@ struct Davidson {
     // more than half the squared norm) and normalised, all on the device; the norm
     // pair (before, after) goes to the slot's pair for the host's dependency check
     // and a dependent vector is zeroed.
-    void orthonormalize_device(int slot) {
+    void orthonormalize_device(int slot, bool have_c1 = false) {
         double *col = V.p + (i64)m * nd;
         double *pair = dv.p + oPAIR + 2 * slot;
         const double *p0 = pb.p + pN0;
         double *p1 = pb.p + pN1, *p2 = pb.p + pN2;
         if (m == 0) {
             k_scale_final<<<(unsigned)nblk, DT, 0, s>>>(col, nd, p0, p0, p0, nblk, 0, pair);
+        } else if (m <= 16) {
+            // the column is in the basis' sector: the dot products read half the rows
+            if (!have_c1) dots(V.p, m, col, pb.p + pC1, nullptr, nullptr, hnb());
+            update(m, pb.p + pC1, col, p1, nullptr, nullptr);
+            dots(V.p, m, col, pb.p + pC2, p0, p1, hnb());
+            update(m, pb.p + pC2, col, p2, p0, p1);
+            k_scale_final<<<(unsigned)nblk, DT, 0, s>>>(col, nd, p0, p1, p2, nblk, 1, pair);
         } else if (m <= kCgsMax) {
             const dim3 grid((unsigned)nblk, (unsigned)((m + 7) / 8));
             k_gram<<<grid, DT, 0, s>>>(nd, V.p, nd, m, col, nd, 1, pb.p + pC1, nullptr, nullptr);
@@ -703,10 +975,34 @@ struct Davidson {
             for (int i = 0; i < m; ++i) hb.p[oY + (size_t)j * m + i] = y[(size_t)i * m + j];
         for (int j = 0; j < k; ++j) hb.p[oTH + j] = theta[(size_t)j];
         PE_HIP(hipMemcpyAsync(dv.p + oY, hb.p + oY, sizeof(double) * (oTH + 16 - oY), hipMemcpyHostToDevice, s));
-        k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, m, dv.p + oY, dv.p + oTH, k, op->diag.p,
+        k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, m, dv.p + oY, dv.p + oTH, k, dg,
                                             write_xw ? 1 : 0, X.p, WX.p, T.p, pb.p + pRES);
         PE_LAUNCH_CHECK();
         k_reduce_partials<<<(unsigned)k, 256, 0, s>>>(pb.p + pRES, nblk, k, dv.p + oRES);
+        PE_LAUNCH_CHECK();
+    }
+
+    // The speculative one-root pass (k = 1, m < mmax, m <= 16): the correction goes
+    // straight to V[:, m] with its squared norm (pN0) and its first Gram-Schmidt
+    // coefficients (pC1), see k_ritz1.
+    void ritz_spec(const std::vector<double> &y, const std::vector<double> &theta) {
+        for (int i = 0; i < m; ++i) hb.p[oY + i] = y[(size_t)i * m];
+        hb.p[oTH] = theta[0];
+        PE_HIP(hipMemcpyAsync(dv.p + oY, hb.p + oY, sizeof(double) * (oTH + 16 - oY), hipMemcpyHostToDevice, s));
+        double *t = V.p + (i64)m * nd;
+        switch (m) {
+#define PE_R1(M)                                                                                               \
+    case M:                                                                                                    \
+        k_ritz1<M><<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, dv.p + oY, dv.p + oTH, dg, t, pb.p + pRES,     \
+                                                 pb.p + pN0, pb.p + pC1);                                      \
+        break;
+            PE_R1(1) PE_R1(2) PE_R1(3) PE_R1(4) PE_R1(5) PE_R1(6) PE_R1(7) PE_R1(8)
+            PE_R1(9) PE_R1(10) PE_R1(11) PE_R1(12) PE_R1(13) PE_R1(14) PE_R1(15) PE_R1(16)
+#undef PE_R1
+            default: throw EngineError(PYCI_EUNSUPPORTED, "basis too large for the one-column passes");
+        }
+        PE_LAUNCH_CHECK();
+        k_reduce_partials<<<1u, 256, 0, s>>>(pb.p + pRES, nblk, 1, dv.p + oRES);
         PE_LAUNCH_CHECK();
     }
 
@@ -719,11 +1015,27 @@ struct Davidson {
             PE_HIP(hipMemcpyAsync(WX.p, W.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
             return;
         }
-        k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, mx, dv.p + oY, dv.p + oTH, k, op->diag.p, 1, X.p,
+        k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, mx, dv.p + oY, dv.p + oTH, k, dg, 1, X.p,
                                             WX.p, T.p, pb.p + pRES);
         PE_LAUNCH_CHECK();
     }
     int mx = 0;  // basis size of the coefficients on the device (for finish_x)
+
+    // V[:, 0:cnt] = src[:, 0:cnt], in a sector projected onto it. The basis stays in
+    // its sector to the last bit by construction (every pass treats a row and its
+    // partner alike); the projection at each restart bounds what rounding could
+    // leave outside it, which the sector sigma does not see and the preconditioner
+    // would amplify, to one restart cycle.
+    void put_basis(const double *src, int cnt) {
+        if (sector == 0) {
+            PE_HIP(hipMemcpyAsync(V.p, src, sizeof(double) * nd * cnt, hipMemcpyDeviceToDevice, s));
+            return;
+        }
+        for (int j = 0; j < cnt; ++j) {
+            k_parity<<<g1(), 256, 0, s>>>(src + (i64)j * nd, V.p + (i64)j * nd, nd, swap_nb, (double)sector);
+            PE_LAUNCH_CHECK();
+        }
+    }
 
     // Thick restart on the lowest kk Ritz vectors: they are orthonormal and G
     // becomes diag(w).
@@ -740,7 +1052,7 @@ struct Davidson {
             PE_LAUNCH_CHECK();
             k_gemm_set<<<g1(), 256, 0, s>>>(nd, W.p, nd, m, dv.p + oYK, kk, RW.p, nd);
             PE_LAUNCH_CHECK();
-            PE_HIP(hipMemcpyAsync(V.p, RV.p, sizeof(double) * nd * kk, hipMemcpyDeviceToDevice, s));
+            put_basis(RV.p, kk);
             PE_HIP(hipMemcpyAsync(W.p, RW.p, sizeof(double) * nd * kk, hipMemcpyDeviceToDevice, s));
         } else {
             // the Ritz vectors of this iteration are the whole restart
@@ -748,7 +1060,7 @@ struct Davidson {
             PE_LAUNCH_CHECK();
             k_gemm_set<<<g1(), 256, 0, s>>>(nd, W.p, nd, m, dv.p + oY, k, WX.p, nd);
             PE_LAUNCH_CHECK();
-            PE_HIP(hipMemcpyAsync(V.p, X.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
+            put_basis(X.p, k);
             PE_HIP(hipMemcpyAsync(W.p, WX.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
         }
         std::fill(G.begin(), G.end(), 0.0);
@@ -773,20 +1085,27 @@ struct Davidson {
                 for (int j = 0; j < m; ++j) gm[(size_t)i * m + j] = G[(size_t)i * mmax + j];
             jacobi_eig(m, gm, w, y);
             theta.assign(w.begin(), w.begin() + k);
-            ritz(y, theta, false);
+            // one root with room for the next column: the correction comes out of the
+            // Ritz pass already in its basis slot (in a sector only where the sigma keeps
+            // the sector to the last bit: there is no projection on this path)
+            const bool fused = spec && exact && m < nd && m + 1 <= mmax && m <= 16;
+            if (fused)
+                ritz_spec(y, theta);
+            else
+                ritz(y, theta, false);
             mx = m;
             bool restarted = false;
             int m0 = m, nspec = 0;
             if (spec && m < nd) {
-                if (m + 1 > mmax) {
+                if (!fused && m + 1 > mmax) {
                     restart(std::max(k, std::min({m, 2 * k, mmax - 1})), w, y);
                     restarted = true;
                 }
                 m0 = m;
-                put_correction(0);
-                orthonormalize_device(0);
+                if (!fused) put_correction(0);
+                orthonormalize_device(0, fused);
                 apply(m0, 1, fp32);
-                gram(V.p, m0 + 1, W.p + (i64)m0 * nd, 1, dv.p + oG);
+                dots_to(V.p, m0 + 1, W.p + (i64)m0 * nd, dv.p + oG, sector != 0 && exact);
                 nspec = 1;
             }
             readback(nspec);
@@ -850,7 +1169,7 @@ struct Davidson {
                     put_correction(add[0]);
                     orthonormalize_device(0);
                     apply(m0, 1, fp32);
-                    gram(V.p, m, W.p + (i64)m0 * nd, 1, dv.p + oG);
+                    dots_to(V.p, m, W.p + (i64)m0 * nd, dv.p + oG, sector != 0 && exact);
                     readback(1);
                     accept_G(m0, 1);
                 } else if (room > 1) {
@@ -893,8 +1212,7 @@ static bool solve_run(pyci_op *op, int k, const double *c0, i64 mm, i64 maxit, d
     const i64 nb = op->swap_nb;
     auto partner = [&](i64 J) { return (J % nb) * nb + J / nb; };
     Davidson D(op, k, (int)mm);
-    D.sector = sector;
-    D.swap_nb = nb;
+    D.set_sector(sector, nb);
     std::vector<double> hd((size_t)nd);
     PE_HIP(hipMemcpyAsync(hd.data(), op->diag.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
