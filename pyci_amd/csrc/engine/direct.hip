@@ -151,6 +151,11 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
     constexpr int MTL = NP ? (NP + 15) / 16 : MT;   // row tiles that hold a pair
     constexpr int BR = VC + 1, LS = VC + 1, NBUF = SB ? 1 : 2;
     static_assert(!(SB && VM == VM_GLOBAL), "one buffer only with V in LDS");
+    // With V in LDS and a free padding row VC - 1 in the k extent (every pair
+    // dimension but 120), the whole update E = V D / 2 + k x is the product: the LDS
+    // copy of V is scaled by 1/2 (exact) and holds k in column VC - 1, and row VC - 1
+    // of the B operand is each column's own X value, so the epilogue only stores.
+    constexpr bool FOLD = VM == VM_FULL && (NP == 0 || NP < VC);
     extern __shared__ unsigned char smem_raw[];
     double *buf0 = reinterpret_cast<double *>(smem_raw);     // NBUF buffers of BR x ld
     T *ks = reinterpret_cast<T *>(buf0 + NBUF * BR * A.ld);  // k[p], kpad values
@@ -162,7 +167,10 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
     for (int i = threadIdx.x; i < NBUF * BR * ld; i += NT) buf0[i] = 0.0;
     for (int i = threadIdx.x; i < KP; i += NT) ks[i] = kv[i];
     if constexpr (VM == VM_FULL)
-        for (int i = threadIdx.x; i < KP * VC; i += NT) vs[(i / VC) * LS + (i % VC)] = V[(i / VC) * KP + (i % VC)];
+        for (int i = threadIdx.x; i < KP * VC; i += NT) {
+            const int r = i / VC, q = i % VC;
+            vs[r * LS + q] = !FOLD ? V[r * KP + q] : (q == VC - 1) ? kv[r] : (T)0.5 * V[r * KP + q];
+        }
     // The two roles run separate loops with the same barrier count, so that neither
     // role's registers are live in the other.
     if (wave >= 4) {
@@ -272,6 +280,7 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
                 T ac[MTL], an[MTL], bc, bn;
                 bc = (T)bb[lk * ld];
                 bb[lk * ld] = 0.0;
+                if (FOLD && NQ == 1 && lk == 3) bc = x;
 #pragma unroll
                 for (int m = 0; m < MTL; ++m) ac[m] = aload(m, 0);
 #pragma unroll
@@ -279,6 +288,7 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
                     if (q + 1 < NQ) {
                         bn = (T)bb[(4 * q + 4 + lk) * ld];
                         bb[(4 * q + 4 + lk) * ld] = 0.0;
+                        if (FOLD && q + 2 == NQ && lk == 3) bn = x;  // row VC - 1
 #pragma unroll
                         for (int m = 0; m < MTL; ++m) an[m] = aload(m, q + 1);
                     }
@@ -302,7 +312,10 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int p = (sizeof(T) == 8) ? 16 * m + lk + 4 * r : 16 * m + 4 * lk + r;
-                        w[p * 64] = (T)0.5 * acc[m][r] + ks[p] * x;
+                        if constexpr (FOLD)
+                            w[p * 64] = acc[m][r];
+                        else
+                            w[p * 64] = (T)0.5 * acc[m][r] + ks[p] * x;
                     }
             }
             x = xat(c + 64 * (int)gridDim.x);
@@ -706,6 +719,9 @@ static void launch_dprod(const MfView &M, const MfCall &C, const T *X, const T *
     PE_REQUIRE(!fixed_npair(MT) || M.npair == fixed_npair(MT), PYCI_EUNSUPPORTED,
                "pair count does not match the pair dimension");
     PE_REQUIRE(P::lds <= cap, PYCI_EUNSUPPORTED, "the device has too little LDS for the matrix-free product");
+    // the folded k term needs a zero row of D below the k extent's end
+    PE_REQUIRE(P::VM != VM_FULL || fixed_npair(MT) || M.npair < dprod_vc(MT), PYCI_EUNSUPPORTED,
+               "pair count fills the pair dimension");
     DprodArgs A{(int)(C.nd * C.nvec), (int)M.nbs, C.nvec, M.npair, M.nea, ntot, C.vstride, C.v0, C.colmajor,
                 (int)C.nd, C.dyn, (int)(M.na * M.nea), (int)(M.na * M.nea + M.nbs * M.neb), P::ld};
     if (ntot <= 64) launch_dprod_v<MT, T, P::SB, P::VM, 8>(M, A, X, V, kv, W, P::lds, s);
