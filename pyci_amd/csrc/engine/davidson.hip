@@ -301,21 +301,27 @@ __global__ void __launch_bounds__(DT) k_update_norm_m(i64 nd, const double *__re
     block_sum_to(ss, npart + blockIdx.x);
 }
 
-// One root, the speculative iteration: with the Ritz coefficients y (M) and theta,
+// One root, the speculative iteration: with the Ritz coefficients y (M) and theta
+// (kernel arguments),
 // per row x = V y, wx = W y, res = wx - theta x and the correction
 // t = res / (theta - d) (d the diagonal, the denominator kept off zero as in
 // k_ritz), written to T. Per block: the squared residual norm (pres), the squared
 // norm of t (pnorm) and the first Gram-Schmidt coefficients V^T t (pdots, M per
 // block). With the basis, its images and d all in one parity sector (d symmetric
 // under P), t is in that sector to the last bit, with no projection.
+struct RitzCoef {
+    double y[16];  // Ritz coefficients of the root (the first M used)
+    double theta;
+};
+
 template <int M>
 __global__ void __launch_bounds__(DT) k_ritz1(i64 nd, const double *__restrict__ V, const double *__restrict__ W,
-                                              const double *Y, const double *theta, const double *__restrict__ diag,
-                                              double *__restrict__ T, double *pres, double *pnorm, double *pdots) {
+                                              RitzCoef cf, const double *__restrict__ diag, double *__restrict__ T,
+                                              double *pres, double *pnorm, double *pdots) {
     double y[M];
 #pragma unroll
-    for (int i = 0; i < M; ++i) y[i] = Y[i];
-    const double th = theta[0];
+    for (int i = 0; i < M; ++i) y[i] = cf.y[i];
+    const double th = cf.theta;
     double acc[M + 2];
 #pragma unroll
     for (int i = 0; i < M + 2; ++i) acc[i] = 0.0;
@@ -581,6 +587,9 @@ struct Davidson {
     DevBuf<double> RV, RW;   // thick-restart scratch (nd x 2k each)
     DevBuf<double> dv;       // small device operands and results (layout: the o* offsets)
     HostBuf<double> hb;      // page-locked mirror of dv for the per-iteration transfers
+    double *hd = nullptr;    // hb as the device sees it: the residual norms, norm pairs and
+                             // Rayleigh columns are written there by the kernels themselves
+    bool y_dev = false;      // the Ritz coefficients of the last Ritz pass are in dv
     DevBuf<double> pb;       // per-block partials of the residual norms, the Gram-Schmidt
                              // coefficients and the correction norms (layout: the p* offsets)
     DevBuf<float> row32, out32;
@@ -626,6 +635,7 @@ struct Davidson {
         osz = oG + M * 16;
         dv.alloc(osz);
         hb.alloc(osz);
+        PE_HIP(hipHostGetDevicePointer((void **)&hd, hb.p, 0));
         const size_t NB = (size_t)nblk;
         pRES = 0;
         pN0 = pRES + NB * 16;
@@ -779,7 +789,7 @@ struct Davidson {
     // and a dependent vector is zeroed.
     void orthonormalize_device(int slot, bool have_c1 = false) {
         double *col = V.p + (i64)m * nd;
-        double *pair = dv.p + oPAIR + 2 * slot;
+        double *pair = hd + oPAIR + 2 * slot;
         const double *p0 = pb.p + pN0;
         double *p1 = pb.p + pN1, *p2 = pb.p + pN2;
         if (m == 0) {
@@ -899,17 +909,9 @@ struct Davidson {
         return cnt;
     }
 
-    // Copies the results of this iteration (residual norms; with cnt new columns
-    // their G block and norm pairs) to the host and waits for them.
-    void readback(int cnt, bool pairs = true) {
-        const size_t last = cnt > 0 ? oG + (size_t)m * cnt : oRES + (size_t)k;
-        if (pairs) {
-            PE_HIP(hipMemcpyAsync(hb.p + oRES, dv.p + oRES, sizeof(double) * (last - oRES), hipMemcpyDeviceToHost, s));
-        } else {
-            PE_HIP(hipMemcpyAsync(hb.p + oG, dv.p + oG, sizeof(double) * (last - oG), hipMemcpyDeviceToHost, s));
-        }
-        PE_HIP(hipStreamSynchronize(s));
-    }
+    // Waits for the results of this iteration (residual norms; with new columns their
+    // G block and norm pairs), which the kernels write straight into hb.
+    void readback() { PE_HIP(hipStreamSynchronize(s)); }
 
     // Fills G for the cnt new columns m0..m0+cnt-1 from the read-back block and
     // removes the columns that were zeroed as dependent. Returns the number kept.
@@ -975,10 +977,11 @@ struct Davidson {
             for (int i = 0; i < m; ++i) hb.p[oY + (size_t)j * m + i] = y[(size_t)i * m + j];
         for (int j = 0; j < k; ++j) hb.p[oTH + j] = theta[(size_t)j];
         PE_HIP(hipMemcpyAsync(dv.p + oY, hb.p + oY, sizeof(double) * (oTH + 16 - oY), hipMemcpyHostToDevice, s));
+        y_dev = true;
         k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, m, dv.p + oY, dv.p + oTH, k, dg,
                                             write_xw ? 1 : 0, X.p, WX.p, T.p, pb.p + pRES);
         PE_LAUNCH_CHECK();
-        k_reduce_partials<<<(unsigned)k, 256, 0, s>>>(pb.p + pRES, nblk, k, dv.p + oRES);
+        k_reduce_partials<<<(unsigned)k, 256, 0, s>>>(pb.p + pRES, nblk, k, hd + oRES);
         PE_LAUNCH_CHECK();
     }
 
@@ -986,15 +989,16 @@ struct Davidson {
     // straight to V[:, m] with its squared norm (pN0) and its first Gram-Schmidt
     // coefficients (pC1), see k_ritz1.
     void ritz_spec(const std::vector<double> &y, const std::vector<double> &theta) {
-        for (int i = 0; i < m; ++i) hb.p[oY + i] = y[(size_t)i * m];
-        hb.p[oTH] = theta[0];
-        PE_HIP(hipMemcpyAsync(dv.p + oY, hb.p + oY, sizeof(double) * (oTH + 16 - oY), hipMemcpyHostToDevice, s));
+        RitzCoef cf{};
+        for (int i = 0; i < m; ++i) cf.y[i] = hb.p[oY + i] = y[(size_t)i * m];
+        cf.theta = hb.p[oTH] = theta[0];
+        y_dev = false;  // in hb only (finish_x uploads them when it needs them)
         double *t = V.p + (i64)m * nd;
         switch (m) {
 #define PE_R1(M)                                                                                               \
     case M:                                                                                                    \
-        k_ritz1<M><<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, dv.p + oY, dv.p + oTH, dg, t, pb.p + pRES,     \
-                                                 pb.p + pN0, pb.p + pC1);                                      \
+        k_ritz1<M><<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, cf, dg, t, pb.p + pRES, pb.p + pN0,           \
+                                                 pb.p + pC1);                                                  \
         break;
             PE_R1(1) PE_R1(2) PE_R1(3) PE_R1(4) PE_R1(5) PE_R1(6) PE_R1(7) PE_R1(8)
             PE_R1(9) PE_R1(10) PE_R1(11) PE_R1(12) PE_R1(13) PE_R1(14) PE_R1(15) PE_R1(16)
@@ -1002,7 +1006,7 @@ struct Davidson {
             default: throw EngineError(PYCI_EUNSUPPORTED, "basis too large for the one-column passes");
         }
         PE_LAUNCH_CHECK();
-        k_reduce_partials<<<1u, 256, 0, s>>>(pb.p + pRES, nblk, 1, dv.p + oRES);
+        k_reduce_partials<<<1u, 256, 0, s>>>(pb.p + pRES, nblk, 1, hd + oRES);
         PE_LAUNCH_CHECK();
     }
 
@@ -1014,6 +1018,11 @@ struct Davidson {
             PE_HIP(hipMemcpyAsync(X.p, V.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
             PE_HIP(hipMemcpyAsync(WX.p, W.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
             return;
+        }
+        if (!y_dev) {
+            PE_HIP(hipMemcpyAsync(dv.p + oY, hb.p + oY, sizeof(double) * (oTH + 16 - oY), hipMemcpyHostToDevice,
+                                  s));
+            y_dev = true;
         }
         k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, mx, dv.p + oY, dv.p + oTH, k, dg, 1, X.p,
                                             WX.p, T.p, pb.p + pRES);
@@ -1105,10 +1114,10 @@ struct Davidson {
                 if (!fused) put_correction(0);
                 orthonormalize_device(0, fused);
                 apply(m0, 1, fp32);
-                dots_to(V.p, m0 + 1, W.p + (i64)m0 * nd, dv.p + oG, sector != 0 && exact);
+                dots_to(V.p, m0 + 1, W.p + (i64)m0 * nd, hd + oG, sector != 0 && exact);
                 nspec = 1;
             }
-            readback(nspec);
+            readback();
             std::vector<double> res(k, 0.0);
             bool all = true, floor = true;
             for (int j = 0; j < k; ++j) {
@@ -1169,8 +1178,8 @@ struct Davidson {
                     put_correction(add[0]);
                     orthonormalize_device(0);
                     apply(m0, 1, fp32);
-                    dots_to(V.p, m, W.p + (i64)m0 * nd, dv.p + oG, sector != 0 && exact);
-                    readback(1);
+                    dots_to(V.p, m, W.p + (i64)m0 * nd, hd + oG, sector != 0 && exact);
+                    readback();
                     accept_G(m0, 1);
                 } else if (room > 1) {
                     // block Gram-Schmidt: the basis is read once per pass for all corrections
@@ -1183,8 +1192,8 @@ struct Davidson {
                     m = m0 + kept;
                     if (kept > 0) {
                         apply(m0, kept, fp32);
-                        gram(V.p, m, W.p + (i64)m0 * nd, kept, dv.p + oG);
-                        readback(kept, false);
+                        gram(V.p, m, W.p + (i64)m0 * nd, kept, hd + oG);
+                        readback();
                         accept_G(m0, kept);
                     }
                 }
