@@ -261,6 +261,15 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
             return A.colmajor ? X[v * A.nd + J] : X[J * A.vstride + A.v0 + v];
         };
         T x = xat((int)blockIdx.x * 64 + 16 * wave + lr);
+        // Two-buffer plans with the folded epilogue store a tile's E values during the
+        // next tile's k loop, one per k step after its MFMAs (pv, pw, pok: the values,
+        // their column and whether it exists), so no run of stores stands between the
+        // MFMAs of consecutive tiles; the last tile's values are stored after the loop.
+        constexpr bool PIPE = FOLD && !SB;
+        T pv[MTL][4];
+        T *pw = W;
+        bool pok = false;
+        auto row_of = [&](int m, int r) { return (sizeof(T) == 8) ? 16 * m + lk + 4 * r : 16 * m + 4 * lk + r; };
         auto product = [&](int tile, double *buf) {
             const int c = tile * 64 + 16 * wave + lr;
             double *bb = buf + 16 * wave + lr;
@@ -299,19 +308,32 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
 #pragma unroll
                     for (int m = 0; m < MTL; ++m) acc[m] = mfma16(ac[m], bc, acc[m]);
                     if constexpr (!SB) __builtin_amdgcn_sched_barrier(0);
+                    if constexpr (PIPE)
+                        if (q < MTL * 4 && pok) pw[row_of(q / 4, q % 4) * 64] = pv[q / 4][q % 4];
                     asm volatile("" ::: "memory");
 #pragma unroll
                     for (int m = 0; m < MTL; ++m) ac[m] = an[m];
                     bc = bn;
                 }
             }
-            if (c < A.ncol) {
+            if constexpr (PIPE) {
+                // the previous tile's values beyond the k steps, then this tile's in their place
+#pragma unroll
+                for (int j = NQ; j < MTL * 4; ++j)
+                    if (pok) pw[row_of(j / 4, j % 4) * 64] = pv[j / 4][j % 4];
+#pragma unroll
+                for (int m = 0; m < MTL; ++m)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) pv[m][r] = acc[m][r];
+                pw = W + (i64)tile * KP * 64 + 16 * wave + lr;
+                pok = c < A.ncol;
+            } else if (c < A.ncol) {
                 T *w = W + (i64)tile * KP * 64 + 16 * wave + lr;
 #pragma unroll
                 for (int m = 0; m < MTL; ++m)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const int p = (sizeof(T) == 8) ? 16 * m + lk + 4 * r : 16 * m + 4 * lk + r;
+                        const int p = row_of(m, r);
                         if constexpr (FOLD)
                             w[p * 64] = acc[m][r];
                         else
@@ -327,6 +349,12 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
                     product((int)blockIdx.x + (step - 1) * (int)gridDim.x, buf0 + ((step - 1) & 1) * BR * ld);
                 __syncthreads();
             }
+            if constexpr (PIPE)
+                if (pok)
+#pragma unroll
+                    for (int m = 0; m < MTL; ++m)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) pw[row_of(m, r) * 64] = pv[m][r];
         } else {
             __syncthreads();
             for (int step = 0; step < nmine; ++step) {
