@@ -416,6 +416,40 @@ __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int n
         if (v < nvec) Y[colmajor ? v * nd + I : I * vstride + v0 + v] = acc[v];
 }
 
+// Alpha terms of one row-major vector (the first pass of a sector sigma): one wave per
+// alpha string Ia and 64 consecutive beta strings, so the table row of Ia is the same
+// for the whole wave and comes through scalar loads; each chunk of CH entries issues
+// its CH E loads before adding any. Sums in the order of k_mf_sigma (same bits).
+template <typename T>
+__global__ void __launch_bounds__(256) k_mf_alpha(int na, int NB, int nchunk, int nea, int KP,
+                                                  const int2 *__restrict__ ta, const T *__restrict__ W,
+                                                  T *__restrict__ Y) {
+    const int wv = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int Ia = __builtin_amdgcn_readfirstlane(wv / nchunk);
+    const int q = __builtin_amdgcn_readfirstlane(wv - (wv / nchunk) * nchunk);
+    if (Ia >= na) return;
+    const int Ib = q * 64 + lane;
+    const bool live = Ib < NB;
+    const i64 ib = live ? Ib : NB - 1;
+    const int2 *row = ta + (i64)Ia * nea;
+    constexpr int CH = 16;
+    T acc = (T)0;
+    for (int t0 = 0; t0 < nea; t0 += CH) {
+        int2 e[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) e[u] = row[min(t0 + u, nea - 1)];
+        T w[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) w[u] = W[wt_at((i64)e[u].x * NB + ib, e[u].y & 0xffff, KP)];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            const T sg = (t0 + u >= nea) ? (T)0 : (e[u].y >> 16) ? (T)-1 : (T)1;
+            acc += sg * w[u];
+        }
+    }
+    if (live) Y[(i64)Ia * NB + Ib] = acc;
+}
+
 // Sigma of a vector in a parity sector of the alpha <-> beta exchange P (nup == ndn,
 // J = (Ja, Jb) -> (Jb, Ja); x(P J) = s x(J)): D and E then have the same parity,
 // E[p][(Ia, K)] = s E[p][(K, Ia)], so the beta terms of determinant I equal s times
@@ -779,9 +813,16 @@ static void launch_passes(const MfView &M, const MfCall &C, const T *X, T *W, T 
         // alpha terms (every block of 256 consecutive determinants shares few alpha
         // strings, so their table rows are common and the E reads contiguous), then
         // the parity symmetrisation
-        k_mf_sigma<NV, T, false><<<grid_for(nd, 256), 256, 0, s>>>(nd, M.nbs, M.nea, M.neb, M.kpad, M.tab_a, M.tab_b,
-                                                                   C.nvec, C.vstride, C.v0, 1, C.colmajor, C.dyn, W,
-                                                                   Y);
+        if (C.nvec == 1 && C.vstride == 1 && C.v0 == 0 && !C.colmajor && !C.dyn) {
+            const int nchunk = (int)((M.nbs + 63) / 64);
+            const i64 waves = M.na * nchunk;
+            k_mf_alpha<T><<<(unsigned)((waves + 3) / 4), 256, 0, s>>>((int)M.na, (int)M.nbs, nchunk, M.nea, M.kpad,
+                                                                     M.tab_a, W, Y);
+        } else {
+            k_mf_sigma<NV, T, false><<<grid_for(nd, 256), 256, 0, s>>>(nd, M.nbs, M.nea, M.neb, M.kpad, M.tab_a,
+                                                                       M.tab_b, C.nvec, C.vstride, C.v0, 1,
+                                                                       C.colmajor, C.dyn, W, Y);
+        }
         PE_LAUNCH_CHECK();
         k_mf_symm<T><<<grid_for(nd, 256), 256, 0, s>>>(Y, M.nbs, C.nvec, C.vstride, C.v0, C.colmajor, C.dyn,
                                                        (T)sector);
