@@ -432,7 +432,7 @@ __global__ void __launch_bounds__(256) k_mf_alpha(int na, int NB, int nchunk, in
     const bool live = Ib < NB;
     const i64 ib = live ? Ib : NB - 1;
     const int2 *row = ta + (i64)Ia * nea;
-    constexpr int CH = 16;
+    constexpr int CH = 24;
     T acc = (T)0;
     for (int t0 = 0; t0 < nea; t0 += CH) {
         int2 e[CH];
