@@ -1298,7 +1298,9 @@ static bool solve_run(pyci_op *op, int k, const double *c0, i64 mm, i64 maxit, d
     i64 iters = 0;
     bool ok;
     if (mixed) {
-        D.run(tol, maxit, true, 2e-5, theta, iters);
+        // the FP32 phase runs to 2e-6 |theta| (an FP32 sigma is good to ~1e-7 relative,
+        // so this is well above its floor; a stall of 8 iterations also hands over)
+        D.run(tol, maxit, true, 2e-6, theta, iters);
         // FP64 residual-correction phase: restart on the FP32-phase Ritz vectors
         // with exact FP64 images.
         PE_HIP(hipMemcpyAsync(D.V.p, D.X.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
