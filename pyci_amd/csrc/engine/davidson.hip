@@ -263,12 +263,21 @@ __global__ void __launch_bounds__(DT) k_dots(i64 nd, const double *__restrict__ 
 }
 
 // t -= V[:, 0:M] c, c the sums of cpart (M per block), and the partial squared
-// norms of the result per block; gated like k_update_norm.
+// norms of the result per block; gated like k_update_norm. With rout, block 0 also
+// sums the one-value-per-block partials rpart into *rout (the residual norm of the
+// Ritz pass, so that it needs no launch of its own).
 template <int M>
 __global__ void __launch_bounds__(DT) k_update_norm_m(i64 nd, const double *__restrict__ V,
                                                       const double *cpart, double *t, double *npart,
-                                                      const double *gate0, const double *gate1) {
+                                                      const double *gate0, const double *gate1,
+                                                      const double *rpart, double *rout) {
     if (gate0 && !second_pass(gate0, gate1, gridDim.x)) return;
+    if (rout && blockIdx.x == 0) {
+        __shared__ double rs[1];
+        const double *const rp[1] = {rpart};
+        sum_partials<1>(rp, gridDim.x, rs);
+        if (threadIdx.x == 0) *rout = rs[0];
+    }
     __shared__ double c[M];
     reduce_partials_block(cpart, gridDim.x, M, c);
     double cr[M];
@@ -683,11 +692,13 @@ struct Davidson {
     }
 
     // t -= V[:, 0:m] c (c from the partials cp), the partial squared norms to np; m <= 16.
-    void update(int m_, const double *cp, double *t, double *np, const double *g0, const double *g1) {
+    // With ro, the per-block partials rp are summed into *ro as well.
+    void update(int m_, const double *cp, double *t, double *np, const double *g0, const double *g1,
+                const double *rp = nullptr, double *ro = nullptr) {
         switch (m_) {
 #define PE_UPD(M)                                                                                   \
     case M:                                                                                         \
-        k_update_norm_m<M><<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, cp, t, np, g0, g1);             \
+        k_update_norm_m<M><<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, cp, t, np, g0, g1, rp, ro);     \
         break;
             PE_UPD(1) PE_UPD(2) PE_UPD(3) PE_UPD(4) PE_UPD(5) PE_UPD(6) PE_UPD(7) PE_UPD(8)
             PE_UPD(9) PE_UPD(10) PE_UPD(11) PE_UPD(12) PE_UPD(13) PE_UPD(14) PE_UPD(15) PE_UPD(16)
@@ -797,7 +808,11 @@ struct Davidson {
         } else if (m <= 16) {
             // the column is in the basis' sector: the dot products read half the rows
             if (!have_c1) dots(V.p, m, col, pb.p + pC1, nullptr, nullptr, hnb());
-            update(m, pb.p + pC1, col, p1, nullptr, nullptr);
+            // after the fused Ritz pass, the first update also finishes its residual norm
+            if (have_c1)
+                update(m, pb.p + pC1, col, p1, nullptr, nullptr, pb.p + pRES, hd + oRES);
+            else
+                update(m, pb.p + pC1, col, p1, nullptr, nullptr);
             dots(V.p, m, col, pb.p + pC2, p0, p1, hnb());
             update(m, pb.p + pC2, col, p2, p0, p1);
             k_scale_final<<<(unsigned)nblk, DT, 0, s>>>(col, nd, p0, p1, p2, nblk, 1, pair);
@@ -1005,9 +1020,7 @@ struct Davidson {
 #undef PE_R1
             default: throw EngineError(PYCI_EUNSUPPORTED, "basis too large for the one-column passes");
         }
-        PE_LAUNCH_CHECK();
-        k_reduce_partials<<<1u, 256, 0, s>>>(pb.p + pRES, nblk, 1, hd + oRES);
-        PE_LAUNCH_CHECK();
+        PE_LAUNCH_CHECK();  // its residual norm is summed by the first Gram-Schmidt update
     }
 
     // Ritz vectors X, WX of the pairs whose residuals were just read: recomputed

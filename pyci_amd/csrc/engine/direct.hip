@@ -265,6 +265,7 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
         // next tile's k loop, one per k step after its MFMAs (pv, pw, pok: the values,
         // their column and whether it exists), so no run of stores stands between the
         // MFMAs of consecutive tiles; the last tile's values are stored after the loop.
+        // Rows past the last pair are not stored (no pass reads them).
         constexpr bool PIPE = FOLD && !SB;
         T pv[MTL][4];
         T *pw = W;
@@ -309,7 +310,8 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
                     for (int m = 0; m < MTL; ++m) acc[m] = mfma16(ac[m], bc, acc[m]);
                     if constexpr (!SB) __builtin_amdgcn_sched_barrier(0);
                     if constexpr (PIPE)
-                        if (q < MTL * 4 && pok) pw[row_of(q / 4, q % 4) * 64] = pv[q / 4][q % 4];
+                        if (q < MTL * 4 && pok && row_of(q / 4, q % 4) < A.npair)
+                            pw[row_of(q / 4, q % 4) * 64] = pv[q / 4][q % 4];
                     asm volatile("" ::: "memory");
 #pragma unroll
                     for (int m = 0; m < MTL; ++m) ac[m] = an[m];
@@ -320,7 +322,7 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
                 // the previous tile's values beyond the k steps, then this tile's in their place
 #pragma unroll
                 for (int j = NQ; j < MTL * 4; ++j)
-                    if (pok) pw[row_of(j / 4, j % 4) * 64] = pv[j / 4][j % 4];
+                    if (pok && row_of(j / 4, j % 4) < A.npair) pw[row_of(j / 4, j % 4) * 64] = pv[j / 4][j % 4];
 #pragma unroll
                 for (int m = 0; m < MTL; ++m)
 #pragma unroll
@@ -354,7 +356,8 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
 #pragma unroll
                     for (int m = 0; m < MTL; ++m)
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) pw[row_of(m, r) * 64] = pv[m][r];
+                        for (int r = 0; r < 4; ++r)
+                            if (row_of(m, r) < A.npair) pw[row_of(m, r) * 64] = pv[m][r];
         } else {
             __syncthreads();
             for (int step = 0; step < nmine; ++step) {
