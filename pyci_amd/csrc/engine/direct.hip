@@ -484,6 +484,32 @@ __global__ void k_mf_symm(T *Y, i64 NB, int nvec, int vstride, int v0, int colma
     }
 }
 
+// k_mf_symm for one row-major vector, by tiles: block b owns the 32 x 32 tile (ti, tj)
+// of (alpha, beta) strings with ti <= tj and its mirror (tj, ti), read into LDS with
+// coalesced rows and written back the same way (the same sums as k_mf_symm).
+template <typename T>
+__global__ void __launch_bounds__(256) k_mf_symm_tiles(T *Y, int NB, T s) {
+    __shared__ T A[32][33], B[32][33];
+    const int b = (int)blockIdx.x;
+    int tj = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
+    while (tj > 0 && tj * (tj + 1) / 2 > b) --tj;
+    while ((tj + 1) * (tj + 2) / 2 <= b) ++tj;
+    const int ti = b - tj * (tj + 1) / 2;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int ia = ti * 32 + r, ib = tj * 32 + tx, ja = tj * 32 + r, jb = ti * 32 + tx;
+        A[r][tx] = (ia < NB && ib < NB) ? Y[(i64)ia * NB + ib] : (T)0;
+        B[r][tx] = (ja < NB && jb < NB) ? Y[(i64)ja * NB + jb] : (T)0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int ia = ti * 32 + r, ib = tj * 32 + tx, ja = tj * 32 + r, jb = ti * 32 + tx;
+        // Y(ia, ib) pairs with Y(ib, ia) = B[tx][r]; Y(ja, jb) with Y(jb, ja) = A[tx][r]
+        if (ia < NB && ib < NB) Y[(i64)ia * NB + ib] = A[r][tx] + s * B[tx][r];
+        if (ti != tj && ja < NB && jb < NB) Y[(i64)ja * NB + jb] = B[r][tx] + s * A[tx][r];
+    }
+}
+
 __global__ void k_direct_fill(double *x, i64 n, u64 seed) {
     for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
         x[i] = (double)(mix64(seed + (u64)i) >> 11) * 0x1p-53 - 0.5;
@@ -827,8 +853,13 @@ static void launch_passes(const MfView &M, const MfCall &C, const T *X, T *W, T 
                                                                        C.colmajor, C.dyn, W, Y);
         }
         PE_LAUNCH_CHECK();
-        k_mf_symm<T><<<grid_for(nd, 256), 256, 0, s>>>(Y, M.nbs, C.nvec, C.vstride, C.v0, C.colmajor, C.dyn,
-                                                       (T)sector);
+        if (C.nvec == 1 && C.vstride == 1 && C.v0 == 0 && !C.colmajor && !C.dyn) {
+            const i64 nt = (M.nbs + 31) / 32;
+            k_mf_symm_tiles<T><<<(unsigned)(nt * (nt + 1) / 2), 256, 0, s>>>(Y, (int)M.nbs, (T)sector);
+        } else {
+            k_mf_symm<T><<<grid_for(nd, 256), 256, 0, s>>>(Y, M.nbs, C.nvec, C.vstride, C.v0, C.colmajor, C.dyn,
+                                                           (T)sector);
+        }
         PE_LAUNCH_CHECK();
         if (ev) PE_HIP(hipEventRecord(ev[2], s));
         return;
