@@ -167,6 +167,15 @@ void pyci_op_destroy(pyci_op *op);
  * float64 in host memory. */
 int pyci_add_hci(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs, double eps, int64_t *nadded);
 
+/* compute_enpt2 (enpt2.cpp:342-370, per-det generator :76-235 FullCI and
+ * :257-328 GenCI; a DOCI set enters as its FullCI set of (d, d) determinants,
+ * :382-386): pt_energy = energy + sum_a (sum_i H_ai c_i)^2 / (energy - ecore - H_aa)
+ * over the determinants a outside the set reached from some det i by an
+ * excitation with |H_ai| > eps/|c_i| (the add_hci screening). coeffs: ndet
+ * float64, host memory. nterms (may be NULL): number of external determinants. */
+int pyci_compute_enpt2(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs, double energy, double eps,
+                       double *pt_energy, int64_t *nterms);
+
 /* compute_rdms (rdm.cpp:1020-1056). coeffs: ndet float64, host memory.
  *   DOCI:   rdm1 = d0 (n,n),   rdm2 = d2 (n,n)          (rdm.cpp:20-65)
  *   FullCI: rdm1 = (2,n,n),    rdm2 = (3,n,n,n,n)      (rdm.cpp:269-530)

@@ -66,6 +66,9 @@ def lib():
         L.oracle_rdms_fullci.argtypes = [i64, i64, u64p, dp, dp, dp]
         L.oracle_rdms_genci.restype = None
         L.oracle_rdms_genci.argtypes = [i64, i64, u64p, dp, dp, dp]
+        L.oracle_enpt2.restype = ctypes.c_double
+        L.oracle_enpt2.argtypes = [ctypes.c_int, i64, i64, i64, i64, u64p, dp, dp, dp, ctypes.c_double,
+                                   ctypes.c_double, i64p]
         L.oracle_diagonal.restype = None
         L.oracle_diagonal.argtypes = [ctypes.c_int, i64, i64, i64, i64, u64p, dp, dp, dp]
         _lib = L
@@ -223,6 +226,25 @@ def diagonal(kind, nbasis, nup, ndn, dets, one_mo, two_mo):
     d = np.zeros(ndet)
     lib().oracle_diagonal(KIND[kind], nbasis, nup, ndn, ndet, dp, op1, op2, _f64(d)[1])
     return d
+
+
+def enpt2(kind, nbasis, nup, ndn, dets, one_mo, two_mo, coeffs, energy, eps, ecore=0.0):
+    """compute_enpt2 restated (enpt2.cpp:342-370): energy + sum_a (sum_i H_ai c_i)^2 /
+    (energy - ecore - H_aa); a DOCI set enters as its FullCI (d, d) set
+    (enpt2.cpp:382-386). Returns (energy, number of external determinants)."""
+    if kind == "doci":
+        d = np.asarray(dets, dtype=np.uint64).reshape(-1)
+        dets = np.stack([d, d], axis=1)
+        kind, nup, ndn = "fullci", nup, nup
+    dets, dp = _u64(np.asarray(dets).reshape(-1))
+    ndet = dets.size // nkey(kind)
+    one_mo, op1 = _f64(one_mo)
+    two_mo, op2 = _f64(two_mo)
+    coeffs, cp = _f64(coeffs)
+    nt = np.zeros(1, dtype=np.int64)
+    corr = lib().oracle_enpt2(KIND[kind], nbasis, nup, ndn, ndet, dp, op1, op2, cp, float(energy - ecore),
+                              float(eps), _i64(nt))
+    return energy + corr, int(nt[0])
 
 
 def lowest_eigs(csr, n=1, tol=1e-12):

@@ -988,3 +988,248 @@ void oracle_diagonal(int kind, i64 nbasis, i64 nup, i64 ndn, i64 ndet, const u64
         oracle_csr_free(c);
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Threaded symmetric product for config-sized golden vectors               */
+/* ------------------------------------------------------------------------ */
+
+/* perform_op_symm (sparseop.cpp:148-159) over a lower+diagonal CSR held as
+ * int64 row pointers, int32 columns and float64 values (nrow rows, row i global
+ * row i): y = (L + L^T - D) x. The rows are split into nthread ranges of equal
+ * nonzero count; each range accumulates into a private y (its own rows and the
+ * transposed half), and the private vectors are summed in range order, so the
+ * result does not depend on thread timing. ecore is not added. */
+#include <pthread.h>
+
+typedef struct {
+    i64 r0, r1, n;
+    const i64 *indptr;
+    const int32_t *indices;
+    const double *data, *x;
+    double *y;
+} symv_job;
+
+static void *symv_worker(void *arg) {
+    symv_job *J = (symv_job *)arg;
+    for (i64 i = J->r0; i < J->r1; ++i) {
+        double acc = 0.0, xi = J->x[i];
+        for (i64 p = J->indptr[i]; p < J->indptr[i + 1]; ++p) {
+            i64 j = J->indices[p];
+            acc += J->data[p] * J->x[j];
+            if (j != i) J->y[j] += J->data[p] * xi;
+        }
+        J->y[i] += acc;
+    }
+    return NULL;
+}
+
+int oracle_symv_lower_mt(i64 nrow, const i64 *indptr, const int32_t *indices, const double *data, const double *x,
+                         double *y, int nthread) {
+    if (nthread < 1) nthread = 1;
+    if (nthread > 64) nthread = 64;
+    symv_job jobs[64];
+    pthread_t th[64];
+    i64 nnz = indptr[nrow], r = 0;
+    for (int t = 0; t < nthread; ++t) {
+        i64 target = nnz / nthread * (t + 1), r1 = r;
+        if (t == nthread - 1) r1 = nrow;
+        else while (r1 < nrow && indptr[r1] < target) ++r1;
+        jobs[t].r0 = r;
+        jobs[t].r1 = r1;
+        jobs[t].n = nrow;
+        jobs[t].indptr = indptr;
+        jobs[t].indices = indices;
+        jobs[t].data = data;
+        jobs[t].x = x;
+        jobs[t].y = (double *)calloc((size_t)nrow, sizeof(double));
+        if (!jobs[t].y) return -1;
+        r = r1;
+    }
+    for (int t = 0; t < nthread; ++t) pthread_create(&th[t], NULL, symv_worker, &jobs[t]);
+    for (int t = 0; t < nthread; ++t) pthread_join(th[t], NULL);
+    for (i64 i = 0; i < nrow; ++i) {
+        double s = 0.0;
+        for (int t = 0; t < nthread; ++t) s += jobs[t].y[i];
+        y[i] = s;
+    }
+    for (int t = 0; t < nthread; ++t) free(jobs[t].y);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Epstein-Nesbet PT2 (enpt2.cpp:76-235 FullCI, :257-328 GenCI, :342-370)    */
+/* ------------------------------------------------------------------------ */
+
+/* Diagonal element of one determinant (compute_enpt2_thread_gather,
+ * enpt2.cpp:37-70 FullCI, :237-255 GenCI): one_mo[j,j], same-spin pairs
+ * <jl|jl> - <jl|lj>, opposite-spin pairs <jl|jl>, in the reference's order. */
+static double pt2_diag(int kind, const hamv *H, const u64 *k) {
+    i64 n1 = H->n1, n2 = n1 * n1, n3 = n2 * n1;
+    i64 ou[64], od[64];
+    int nou = occs_of(k[0], ou), nod = (kind == KIND_FULLCI) ? occs_of(k[1], od) : 0;
+    double diag = 0.0;
+    for (int i = 0; i < nou; ++i) {
+        i64 j = ou[i];
+        diag += H->one_mo[(n1 + 1) * j];
+        for (int q = i + 1; q < nou; ++q) {
+            i64 l = ou[q];
+            diag += G(j, l, j, l) - G(j, l, l, j);
+        }
+        for (int q = 0; q < nod; ++q) {
+            i64 l = od[q];
+            diag += G(j, l, j, l);
+        }
+    }
+    for (int i = 0; i < nod; ++i) {
+        i64 j = od[i];
+        diag += H->one_mo[(n1 + 1) * j];
+        for (int q = i + 1; q < nod; ++q) {
+            i64 l = od[q];
+            diag += G(j, l, j, l) - G(j, l, l, j);
+        }
+    }
+    return diag;
+}
+
+typedef struct {
+    dset t;        /* external determinants in first-occurrence order */
+    double *num;   /* sum_i H_ai c_i */
+    double *diag;  /* H_aa */
+    i64 cap;
+} pt2_terms;
+
+static void pt2_add(pt2_terms *P, int kind, const hamv *H, const dset *s, const u64 *k, double v) {
+    if (dset_find(s, k) != -1) return; /* index_det_from_rank(rank) == -1 only */
+    i64 a = dset_find(&P->t, k);
+    if (a < 0) {
+        a = dset_add(&P->t, k);
+        if (a >= P->cap) {
+            P->cap = 2 * (a + 1);
+            P->num = (double *)realloc(P->num, sizeof(double) * P->cap);
+            P->diag = (double *)realloc(P->diag, sizeof(double) * P->cap);
+        }
+        P->num[a] = 0.0;
+        P->diag[a] = pt2_diag(kind, H, k);
+    }
+    P->num[a] += v;
+}
+
+/* compute_enpt2 (enpt2.cpp:342-370) without the final "energy +": returns
+ * sum_a (sum_i H_ai c_i)^2 / (e - H_aa) over the determinants a outside the set
+ * reached from some i by an excitation with |H_ai| > eps/|c_i| (e = energy -
+ * ecore). FullCI (kind 1) and GenCI (kind 2); a DOCI set enters as the FullCI set
+ * of its (d, d) determinants (enpt2.cpp:382-386). The contributions of one
+ * external determinant are summed in serial order (source ascending, reference
+ * loop order); the reference sums per thread and then merges. */
+double oracle_enpt2(int kind, i64 nbasis, i64 nup, i64 ndn, i64 ndet, const u64 *dets, const double *one_mo,
+                    const double *two_mo, const double *coeffs, double e, double eps, i64 *nterms) {
+    int nkey = (kind == KIND_FULLCI) ? 2 : 1;
+    dset s;
+    dset_from(&s, nkey, ndet, dets);
+    hamv H;
+    ham_init(&H, kind, nbasis, nup, ndn, one_mo, two_mo);
+    const hamv *HH = &H;
+    pt2_terms P;
+    dset_init(&P.t, nkey, 1024);
+    P.cap = 1024;
+    P.num = (double *)malloc(sizeof(double) * P.cap);
+    P.diag = (double *)malloc(sizeof(double) * P.cap);
+    i64 n1 = nbasis, n2 = n1 * n1, n3 = n2 * n1;
+    i64 occs[64], virs[64], ou[64], od[64], vu[64], vd[64];
+#define GG(p, q, r, s_) (HH->two_mo[(p) * n3 + (q) * n2 + (r) * n1 + (s_)])
+    for (i64 idet = 0; idet < ndet; ++idet) {
+        const double c = coeffs[idet], eps_i = eps / fabs(c);
+        if (kind == KIND_GENCI) {
+            u64 det = dets[idet];
+            int no = occs_of(det, occs), nv = virs_of(det, n1, virs);
+            for (int i = 0; i < no; ++i) {
+                i64 ii = occs[i];
+                for (int j = 0; j < nv; ++j) {
+                    i64 jj = virs[j];
+                    u64 ex = (det & ~(1ULL << ii)) | (1ULL << jj);
+                    double val = one_mo[n1 * ii + jj];
+                    for (int k = 0; k < no; ++k) val += GG(ii, occs[k], jj, occs[k]) - GG(ii, occs[k], occs[k], jj);
+                    if (fabs(val) > eps_i) pt2_add(&P, kind, HH, &s, &ex, val * oracle_phase_single(det, ii, jj) * c);
+                    for (int k = i + 1; k < no; ++k)
+                        for (int l = j + 1; l < nv; ++l) {
+                            i64 kk = occs[k], ll = virs[l];
+                            val = GG(ii, kk, jj, ll) - GG(ii, kk, ll, jj);
+                            if (fabs(val) > eps_i) {
+                                u64 e2 = (ex & ~(1ULL << kk)) | (1ULL << ll);
+                                pt2_add(&P, kind, HH, &s, &e2, val * oracle_phase_double(det, ii, kk, jj, ll) * c);
+                            }
+                        }
+                }
+            }
+            continue;
+        }
+        u64 up = dets[2 * idet], dn = dets[2 * idet + 1];
+        int nou = occs_of(up, ou), nod = occs_of(dn, od), nvu = virs_of(up, n1, vu), nvd = virs_of(dn, n1, vd);
+        for (int i = 0; i < nou; ++i) {
+            i64 ii = ou[i];
+            for (int j = 0; j < nvu; ++j) {
+                i64 jj = vu[j];
+                u64 eu = (up & ~(1ULL << ii)) | (1ULL << jj);
+                i64 sign_up = oracle_phase_single(up, ii, jj);
+                double val = one_mo[n1 * ii + jj];
+                for (int k = 0; k < nou; ++k) val += GG(ii, ou[k], jj, ou[k]) - GG(ii, ou[k], ou[k], jj);
+                for (int k = 0; k < nod; ++k) val += GG(ii, od[k], jj, od[k]);
+                if (fabs(val) > eps_i) {
+                    u64 key[2] = {eu, dn};
+                    pt2_add(&P, kind, HH, &s, key, val * sign_up * c);
+                }
+                for (int k = 0; k < nod; ++k)
+                    for (int l = 0; l < nvd; ++l) {
+                        i64 kk = od[k], ll = vd[l];
+                        val = GG(ii, kk, jj, ll);
+                        if (fabs(val) > eps_i) {
+                            u64 key[2] = {eu, (dn & ~(1ULL << kk)) | (1ULL << ll)};
+                            pt2_add(&P, kind, HH, &s, key, val * (sign_up * oracle_phase_single(dn, kk, ll)) * c);
+                        }
+                    }
+                for (int k = i + 1; k < nou; ++k)
+                    for (int l = j + 1; l < nvu; ++l) {
+                        i64 kk = ou[k], ll = vu[l];
+                        val = GG(ii, kk, jj, ll) - GG(ii, kk, ll, jj);
+                        if (fabs(val) > eps_i) {
+                            u64 key[2] = {(eu & ~(1ULL << kk)) | (1ULL << ll), dn};
+                            pt2_add(&P, kind, HH, &s, key, val * oracle_phase_double(up, ii, kk, jj, ll) * c);
+                        }
+                    }
+            }
+        }
+        for (int i = 0; i < nod; ++i) {
+            i64 ii = od[i];
+            for (int j = 0; j < nvd; ++j) {
+                i64 jj = vd[j];
+                u64 ed = (dn & ~(1ULL << ii)) | (1ULL << jj);
+                double val = one_mo[n1 * ii + jj];
+                for (int k = 0; k < nou; ++k) val += GG(ii, ou[k], jj, ou[k]);
+                for (int k = 0; k < nod; ++k) val += GG(ii, od[k], jj, od[k]) - GG(ii, od[k], od[k], jj);
+                if (fabs(val) > eps_i) {
+                    u64 key[2] = {up, ed};
+                    pt2_add(&P, kind, HH, &s, key, val * oracle_phase_single(dn, ii, jj) * c);
+                }
+                for (int k = i + 1; k < nod; ++k)
+                    for (int l = j + 1; l < nvd; ++l) {
+                        i64 kk = od[k], ll = vd[l];
+                        val = GG(ii, kk, jj, ll) - GG(ii, kk, ll, jj);
+                        if (fabs(val) > eps_i) {
+                            u64 key[2] = {up, (ed & ~(1ULL << kk)) | (1ULL << ll)};
+                            pt2_add(&P, kind, HH, &s, key, val * oracle_phase_double(dn, ii, kk, jj, ll) * c);
+                        }
+                    }
+            }
+        }
+    }
+#undef GG
+    double corr = 0.0;
+    for (i64 a = 0; a < P.t.n; ++a) corr += P.num[a] * P.num[a] / (e - P.diag[a]);
+    if (nterms) *nterms = P.t.n;
+    free(P.num);
+    free(P.diag);
+    dset_free(&P.t);
+    ham_free(&H);
+    dset_free(&s);
+    return corr;
+}

@@ -205,6 +205,20 @@ static i64 add_hci(SQuantOp &ham, W &wfn, Array<double> coeffs, double eps, i64,
     return added;
 }
 
+template <class W>
+static double enpt2(SQuantOp &ham, W &wfn, Array<double> coeffs, double energy, double eps, i64) {
+    if ((i64)coeffs.size() != wfn.ndet) throw py::value_error("coeffs must have one entry per determinant");
+    if (ham.nbasis != wfn.nbasis) throw py::value_error("hamiltonian and wave function have different nbasis");
+    pyci_wfn *dw = wfn.device();
+    pyci_ham *dh = ham.device();
+    double e = 0.0;
+    {
+        py::gil_scoped_release nogil;
+        check(pyci_compute_enpt2(dh, dw, coeffs.data(), energy, eps, &e, nullptr));
+    }
+    return e;
+}
+
 static py::tuple rdms(Wfn &wfn, Array<double> coeffs) {
     if ((i64)coeffs.size() != wfn.ndet) throw py::value_error("coeffs must have one entry per determinant");
     const i64 n = wfn.nbasis;
@@ -428,6 +442,12 @@ PYBIND11_MODULE(_pyci, m) {
     m.def("add_hci", &add_hci<GenCIWfn>, py::arg("ham"), py::arg("wfn"), py::arg("coeffs"), py::arg("eps") = 1.0e-5,
           py::arg("nthread") = -1, py::arg("selector") = py::none(), py::arg("pool") = 5.0, py::arg("nadd") = -1,
           py::arg("score_mode") = 0);
+    m.def("compute_enpt2", &enpt2<DOCIWfn>, py::arg("ham"), py::arg("wfn"), py::arg("coeffs"), py::arg("energy"),
+          py::arg("eps") = 1.0e-5, py::arg("nthread") = -1);
+    m.def("compute_enpt2", &enpt2<FullCIWfn>, py::arg("ham"), py::arg("wfn"), py::arg("coeffs"), py::arg("energy"),
+          py::arg("eps") = 1.0e-5, py::arg("nthread") = -1);
+    m.def("compute_enpt2", &enpt2<GenCIWfn>, py::arg("ham"), py::arg("wfn"), py::arg("coeffs"), py::arg("energy"),
+          py::arg("eps") = 1.0e-5, py::arg("nthread") = -1);
     m.def("compute_rdms", [](DOCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));
     m.def("compute_rdms", [](FullCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));
     m.def("compute_rdms", [](GenCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));

@@ -535,6 +535,19 @@ int pyci_add_hci(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs, double eps,
     });
 }
 
+int pyci_compute_enpt2(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs, double energy, double eps,
+                       double *pt_energy, int64_t *nterms) {
+    return guarded([&] {
+        PE_REQUIRE(ham->ctx == wfn->ctx, PYCI_EINVAL, "ham and wfn belong to different contexts");
+        PE_REQUIRE(pt_energy != nullptr, PYCI_EINVAL, "null argument");
+        set_device(wfn->ctx);
+        i64 nt = 0;
+        const double corr = enpt2_device(ham, wfn, coeffs, energy - ham->ecore, eps, &nt);
+        *pt_energy = energy + corr;
+        if (nterms) *nterms = nt;
+    });
+}
+
 int pyci_compute_rdms(pyci_wfn *wfn, const double *coeffs, double *rdm1, double *rdm2) {
     return guarded([&] {
         set_device(wfn->ctx);

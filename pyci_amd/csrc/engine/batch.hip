@@ -1085,6 +1085,8 @@ void batch_solve(pyci_ctx *ctx, i64 nprob, const i64 *nbasis, const i64 *nup, co
                                         P.npair, P.kpad, P.nea, P.neb});
             }
         }
+        // mt leaves scope here: a copy from pageable memory may still be pending
+        PE_HIP(hipStreamSynchronize(s));
     }
     B.diag.alloc((size_t)vec);
     B.V.alloc((size_t)basis);
@@ -1201,6 +1203,8 @@ void batch_solve(pyci_ctx *ctx, i64 nprob, const i64 *nbasis, const i64 *nup, co
         PE_HIP(hipMemcpyAsync(dsb.p, sb.data(), sizeof(i64) * sb.size(), hipMemcpyHostToDevice, s));
         PE_HIP(hipMemcpyAsync(dca.p, ca.data(), sizeof(double) * ca.size(), hipMemcpyHostToDevice, s));
         PE_HIP(hipMemcpyAsync(dcb.p, cb.data(), sizeof(double) * cb.size(), hipMemcpyHostToDevice, s));
+        // sa, sb, ca, cb leave scope here: the copies from them may still be pending
+        PE_HIP(hipStreamSynchronize(s));
     }
     {
         std::vector<int> st0((size_t)nsub * ST, 0);

@@ -39,6 +39,8 @@ struct NewSet {
     int *overflow;
     u64 *score = nullptr;  // pool only: max |H_ji c_i| as double bits (nonnegative, so integer max)
     int *plain = nullptr;  // pool only: 1 when some source passes the plain threshold
+    double *num = nullptr; // ENPT2 only: sum_i H_ai c_i of external determinant a
+    u64 *kdet = nullptr;   // ENPT2 only: the determinant words of each claimed slot (2 per slot)
 };
 
 __device__ inline u64 set_key(const WfnView &w, u64 a, u64 b) {
@@ -88,7 +90,8 @@ __device__ inline u64 set_find_pos(const NewSet &S, u64 key) {
 
 #define GIX(p, q, r, s) ((((p) * n + (q)) * n + (r)) * n + (s))
 
-// PASS 1: count, 2: insert, 3: owners per row, 4: write owners.
+// PASS 1: count, 2: insert, 3: owners per row, 4: write owners,
+// 5: ENPT2 numerators (insert, then add the signed H_ai c_i of every passing candidate).
 template <int KIND, int PASS>
 __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double *coeffs, double eps, double eps_plain,
                                              i64 rows_stride, NewSet S, unsigned long long *count, i64 *row_cnt,
@@ -118,7 +121,7 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
     i64 e_base = 0;                // running position inside the row (reference order)
 
     // Evaluates one candidate; returns true when it passes and is new.
-    auto consider = [&](bool valid, u64 nu, u64 nd, double val, u64 pos) {
+    auto consider = [&](bool valid, u64 nu, u64 nd, double val, u64 pos, int sg) {
         bool pass = false;
         u64 key = 0;
         if (valid && fabs(val) > eps_i) {
@@ -129,6 +132,16 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
         }
         if (PASS == 1) {
             local += __popcll(__ballot(pass));
+        } else if (PASS == 5) {
+            // the reference multiplies the element by (phase * c_i) (enpt2.cpp:122-205)
+            if (pass) {
+                const u64 slot = set_insert(S, key, 0);
+                if (slot != ~0ULL) {
+                    atomicAdd(&S.num[slot], val * ((double)sg * coeffs[row]));
+                    S.kdet[2 * slot] = nu;
+                    S.kdet[2 * slot + 1] = nd;
+                }
+            }
         } else if (PASS == 2) {
             if (pass) {
                 const u64 slot = set_insert(S, key, pos);
@@ -159,7 +172,7 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
             const int pi = __shfl(packed, i), pj = __shfl(packed, j);
             const int k = pi & 255, l = (pj >> 8) & 255;
             double val = valid ? two[GIX(k, k, l, l)] : 0.0;
-            consider(valid, up ^ (1ULL << k) ^ (1ULL << l), 0ULL, val, (u64)row * (u64)rows_stride + (u64)e);
+            consider(valid, up ^ (1ULL << k) ^ (1ULL << l), 0ULL, val, (u64)row * (u64)rows_stride + (u64)e, 1);
         }
     } else {
         // alpha (or only) string: for i, for j: single, [1-1 block], same-spin doubles k > i, l > j
@@ -188,7 +201,9 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
                     const int pk = __shfl(packed, k), pl = __shfl(packed, l);
                     u64 nu = up ^ (1ULL << ii) ^ (1ULL << jj), nd = dn;
                     double val = 0.0;
+                    int sg = 1;
                     if (valid) {
+                        if (PASS == 5) sg = phase1(up, ii, jj);
                         if (cls == 0) {
                             double v = one[ii * n + jj];
                             for (u64 x = up; x; x &= x - 1) {
@@ -205,13 +220,15 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
                             const int kk = (pk >> 16) & 255, ll = (pl >> 24) & 255;
                             nd = dn ^ (1ULL << kk) ^ (1ULL << ll);
                             val = two[GIX(ii, kk, jj, ll)];
+                            if (PASS == 5) sg *= phase1(dn, kk, ll);
                         } else {
                             const int kk = pk & 255, ll = (pl >> 8) & 255;
                             nu ^= (1ULL << kk) ^ (1ULL << ll);
                             val = two[GIX(ii, kk, jj, ll)] - two[GIX(ii, kk, ll, jj)];
+                            if (PASS == 5) sg = phase2(up, ii, kk, jj, ll);
                         }
                     }
-                    consider(valid, nu, nd, val, (u64)row * (u64)rows_stride + (u64)(e_base + t));
+                    consider(valid, nu, nd, val, (u64)row * (u64)rows_stride + (u64)(e_base + t), sg);
                 }
                 e_base += bsize;
             }
@@ -234,7 +251,9 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
                         const int pk = __shfl(packed, k), pl = __shfl(packed, l);
                         u64 nd = dn ^ (1ULL << ii) ^ (1ULL << jj);
                         double val = 0.0;
+                        int sg = 1;
                         if (valid) {
+                            if (PASS == 5) sg = phase1(dn, ii, jj);
                             if (t == 0) {
                                 double v = one[ii * n + jj];
                                 for (u64 x = up; x; x &= x - 1) {
@@ -250,9 +269,10 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
                                 const int kk = (pk >> 16) & 255, ll = (pl >> 24) & 255;
                                 nd ^= (1ULL << kk) ^ (1ULL << ll);
                                 val = two[GIX(ii, kk, jj, ll)] - two[GIX(ii, kk, ll, jj)];
+                                if (PASS == 5) sg = phase2(dn, ii, kk, jj, ll);
                             }
                         }
-                        consider(valid, up, nd, val, (u64)row * (u64)rows_stride + (u64)(e_base + t));
+                        consider(valid, up, nd, val, (u64)row * (u64)rows_stride + (u64)(e_base + t), sg);
                     }
                     e_base += bsize;
                 }
@@ -435,6 +455,162 @@ void hci_pool_device(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double eps,
     PE_LAUNCH_CHECK();
     P.nplain = exclusive_scan(wfn->ctx, cnt.p, P.n + 1);
     PE_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// Epstein-Nesbet PT2 (enpt2.cpp:342-370): the same candidate enumeration and
+// screening as add_hci, with every passing external determinant a accumulating
+// sum_i H_ai c_i in a device hash set (pass 5), then
+// sum_a num_a^2 / (e - H_aa) over the set.
+// ---------------------------------------------------------------------------
+
+// H_aa in the reference's order (compute_enpt2_thread_gather, enpt2.cpp:37-70;
+// GenCI :237-255): per occupied j, one_mo[j,j], same-spin pairs, then for FullCI
+// the opposite-spin pairs; then the beta string's own terms.
+__device__ inline double pt2_diag(const HamView &H, u64 up, u64 dn) {
+    const int n = H.n;
+    const double *one = H.one, *two = H.two;
+    double d = 0.0;
+    for (u64 x = up; x; x &= x - 1) {
+        const int j = __builtin_ctzll(x);
+        d += one[j * (n + 1)];
+        for (u64 y = x & (x - 1); y; y &= y - 1) {
+            const int l = __builtin_ctzll(y);
+            d += two[GIX(j, l, j, l)] - two[GIX(j, l, l, j)];
+        }
+        for (u64 y = dn; y; y &= y - 1) {
+            const int l = __builtin_ctzll(y);
+            d += two[GIX(j, l, j, l)];
+        }
+    }
+    for (u64 x = dn; x; x &= x - 1) {
+        const int j = __builtin_ctzll(x);
+        d += one[j * (n + 1)];
+        for (u64 y = x & (x - 1); y; y &= y - 1) {
+            const int l = __builtin_ctzll(y);
+            d += two[GIX(j, l, j, l)] - two[GIX(j, l, l, j)];
+        }
+    }
+    return d;
+}
+
+constexpr int PT2_BLOCKS = 1024;
+
+// partial[b] = block b's sum of num^2 / (e - H_aa) over its slots (grid-stride,
+// then a fixed-order tree in LDS, so the total does not depend on timing).
+__global__ void __launch_bounds__(256) k_pt2_terms(NewSet S, i64 cap, HamView H, double e, double *partial) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (i64 h = (i64)blockIdx.x * 256 + threadIdx.x; h < cap; h += (i64)gridDim.x * 256) {
+        if (S.keys[h] == EMPTY_KEY) continue;
+        const double x = S.num[h];
+        acc += x * x / (e - pt2_diag(H, S.kdet[2 * h], S.kdet[2 * h + 1]));
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_pt2_clear(double *num, u64 *kdet, i64 n) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        num[i] = 0.0;
+        kdet[2 * i] = 0ULL;
+        kdet[2 * i + 1] = 0ULL;
+    }
+}
+
+// (d) -> (d, d): the FullCI set of a DOCI set, in the same order (FullCIWfn(const
+// DOCIWfn &), fullciwfn.cpp:26-35).
+__global__ void k_doci_to_fullci(const u64 *src, i64 n, u64 *dst) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[2 * i] = dst[2 * i + 1] = src[i];
+}
+
+double enpt2_device(pyci_ham *ham, pyci_wfn *wfn_in, const double *coeffs_host, double e, double eps, i64 *nterms) {
+    pyci_ctx *ctx = wfn_in->ctx;
+    hipStream_t s = ctx->stream;
+    PE_REQUIRE(ham->nbasis == wfn_in->nbasis, PYCI_EINVAL, "operator and wave function have different nbasis");
+    if (nterms) *nterms = 0;
+    const i64 nd = wfn_in->ndet;
+    if (nd == 0) return 0.0;
+    // a DOCI set enters as the FullCI set of its (d, d) determinants (enpt2.cpp:382-386)
+    pyci_wfn tmp;
+    pyci_wfn *wfn = wfn_in;
+    if (wfn_in->kind == PYCI_WFN_DOCI) {
+        tmp.ctx = ctx;
+        tmp.kind = PYCI_WFN_FULLCI;
+        tmp.nbasis = wfn_in->nbasis;
+        tmp.nup = tmp.ndn = wfn_in->nup;
+        tmp.nkey = 2;
+        tmp.nsd = binom_host(tmp.nbasis, tmp.ndn);
+        tmp.ndet = nd;
+        tmp.dets.alloc((size_t)(2 * nd));
+        k_doci_to_fullci<<<grid_for(nd, 256), 256, 0, s>>>(wfn_in->dets.p, nd, tmp.dets.p);
+        PE_LAUNCH_CHECK();
+        wfn_build_index(&tmp);
+        wfn = &tmp;
+    }
+    if (wfn->kind == PYCI_WFN_FULLCI) {
+        long double tot = (long double)binom_host(wfn->nbasis, wfn->nup) * (long double)binom_host(wfn->nbasis, wfn->ndn);
+        PE_REQUIRE(tot < 9.0e18L, PYCI_EUNSUPPORTED, "determinant space too large for the GPU selection keys");
+    }
+    DevBuf<double> dc((size_t)nd);
+    PE_HIP(hipMemcpyAsync(dc.p, coeffs_host, sizeof(double) * nd, hipMemcpyHostToDevice, s));
+    const WfnView w = view_of(wfn);
+    const HamView H = view_of(ham);
+    DevBuf<unsigned long long> cnt(2);
+    DevBuf<int> ovf(1);
+    PE_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    NewSet S{nullptr, nullptr, 0, cnt.p + 1, ovf.p};
+    // pass 1: passing external candidates (with repeats) bound the set size
+    launch_hci<1>(wfn->kind, w, H, dc.p, eps, eps, 0, S, cnt.p, nullptr, nullptr, nullptr, s);
+    unsigned long long upper = 0;
+    PE_HIP(hipMemcpyAsync(&upper, cnt.p, sizeof(upper), hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    if (upper == 0) return 0.0;
+    i64 cap = 1024;
+    const i64 want = (i64)std::min<unsigned long long>(upper, 1ULL << 29);
+    while (cap < 2 * want) cap <<= 1;
+    DevBuf<u64> keys, pos, kdet;
+    DevBuf<double> num;
+    unsigned long long uniq = 0;
+    for (;;) {
+        keys.alloc((size_t)cap);
+        pos.alloc((size_t)cap);
+        num.alloc((size_t)cap);
+        kdet.alloc((size_t)(2 * cap));
+        k_set_clear<<<4096, 256, 0, s>>>(keys.p, pos.p, nullptr, nullptr, cap);
+        PE_LAUNCH_CHECK();
+        k_pt2_clear<<<4096, 256, 0, s>>>(num.p, kdet.p, cap);
+        PE_LAUNCH_CHECK();
+        PE_HIP(hipMemsetAsync(cnt.p + 1, 0, sizeof(unsigned long long), s));
+        PE_HIP(hipMemsetAsync(ovf.p, 0, sizeof(int), s));
+        S = NewSet{keys.p, pos.p, (u64)(cap - 1), cnt.p + 1, ovf.p};
+        S.num = num.p;
+        S.kdet = kdet.p;
+        launch_hci<5>(wfn->kind, w, H, dc.p, eps, eps, 0, S, nullptr, nullptr, nullptr, nullptr, s);
+        int of = 0;
+        PE_HIP(hipMemcpyAsync(&uniq, cnt.p + 1, sizeof(uniq), hipMemcpyDeviceToHost, s));
+        PE_HIP(hipMemcpyAsync(&of, ovf.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        PE_HIP(hipStreamSynchronize(s));
+        if (!of && 2 * (i64)uniq <= cap) break;
+        PE_REQUIRE(cap < ((i64)1 << 34), PYCI_ENOMEM, "ENPT2 hash set too large");
+        cap <<= 2;
+    }
+    DevBuf<double> part(PT2_BLOCKS);
+    k_pt2_terms<<<PT2_BLOCKS, 256, 0, s>>>(S, cap, H, e, part.p);
+    PE_LAUNCH_CHECK();
+    std::vector<double> hp(PT2_BLOCKS);
+    PE_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * PT2_BLOCKS, hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    double corr = 0.0;
+    for (double v : hp) corr += v;
+    if (nterms) *nterms = (i64)uniq;
+    return corr;
 }
 
 }  // namespace pe

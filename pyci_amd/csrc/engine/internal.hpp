@@ -329,6 +329,8 @@ void fill_all_dets_device(pyci_ctx *ctx, int kind, i64 nb, i64 nup, i64 ndn, u64
 // HCI / RDMs
 i64 add_hci_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, double eps);
 void rdms_device(pyci_wfn *wfn, const double *coeffs_host, double *rdm1, double *rdm2);
+// ENPT2 correction sum_a (sum_i H_ai c_i)^2 / (e - H_aa), e = energy - ecore (hci.hip)
+double enpt2_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, double e, double eps, i64 *nterms);
 
 // Heat-bath candidate pool (hci.hip): the new determinants passing eps_pool in
 // first-occurrence order, their heat-bath scores max_i |H_ji c_i|, and whether

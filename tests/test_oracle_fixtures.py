@@ -1,0 +1,243 @@
+"""Pins of the oracle's selection, excitation and ENPT2 functions (no GPU):
+  * the reference's own fixtures: tutorial notebook outputs
+    (notebooks/pyci_tutorial.ipynb: HF, CISD 757 dets, HCI 282 dets / 4 iterations),
+    test_run_hci (test_routines.py:431-470) and test_enpt2 (:473-491);
+  * brute force: add_excited_dets as a set from an itertools enumeration, the
+    heat-bath criterion of DOCI and GenCI from the Jordan-Wigner Hamiltonian
+    (oracle/jw.py), and ENPT2 from the same dense Hamiltonian.
+The append order of add_excited_dets is the reference's loop order restated
+(twospinwfn.cpp:247-265, onespinwfn.cpp:219-244); only a run of the reference
+could pin it, so it is recorded as parity unpinned and compared as a set here.
+"""
+import itertools
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+from scipy.special import comb
+
+from oracle import fcidump, jw
+from pyci_amd.synthetic import seeded_integrals, spin_orbital
+from tests.conftest import datafile
+
+
+def _load(name):
+    return fcidump.read(datafile(name + ".fcidump"))
+
+
+def _solve(oracle, kind, nb, nu, nd, dets, one, two):
+    csr = oracle.csr_build(kind, nb, nu, nd, dets, one, two)
+    w, v = oracle.lowest_eigs(csr, 1)
+    return w[0], v[0]
+
+
+def _hf(kind, nu, nd):
+    if kind == "fullci":
+        return np.array([[(1 << nu) - 1, (1 << nd) - 1]], dtype=np.uint64)
+    return np.array([(1 << nu) - 1], dtype=np.uint64)
+
+
+def test_notebook_hf_energy(oracle):
+    """pyci_tutorial.ipynb cell 5: Be/cc-pVDZ HF determinant, E = -14.572337630953388."""
+    ecore, one, two = _load("be_ccpvdz")
+    hf = _hf("fullci", 2, 2)
+    e, _ = _solve(oracle, "fullci", one.shape[0], 2, 2, hf, one, two)
+    npt.assert_allclose(e + ecore, -14.572337630953388, rtol=0, atol=1e-12)
+
+
+def test_notebook_cisd(oracle):
+    """pyci_tutorial.ipynb cell 9: add_excitations(wfn, 0, 1, 2) -> 757 dets, -14.61735579."""
+    ecore, one, two = _load("be_ccpvdz")
+    nb = one.shape[0]
+    hf = _hf("fullci", 2, 2)
+    dets = hf[:0]
+    for e in (0, 1, 2):
+        dets = oracle.add_excited_dets("fullci", nb, 2, 2, dets, hf[0], e)
+    assert len(dets) == 757
+    e, _ = _solve(oracle, "fullci", nb, 2, 2, dets, one, two)
+    npt.assert_allclose(e + ecore, -14.61735579, rtol=0, atol=5e-9)
+
+
+def test_notebook_hci(oracle):
+    """pyci_tutorial.ipynb cell 15: HCI from HF with eps = 5e-4 stops after 4
+    iterations at 282 determinants and -14.61740392."""
+    ecore, one, two = _load("be_ccpvdz")
+    nb = one.shape[0]
+    dets = _hf("fullci", 2, 2)
+    e, c = _solve(oracle, "fullci", nb, 2, 2, dets, one, two)
+    niter, added = 0, 1
+    while added:
+        new = oracle.add_hci("fullci", nb, 2, 2, dets, one, two, c, 5e-4)
+        added = len(new)
+        if added:
+            dets = np.concatenate([dets, new])
+        e, c = _solve(oracle, "fullci", nb, 2, 2, dets, one, two)
+        niter += 1
+    assert niter == 4 and len(dets) == 282
+    npt.assert_allclose(e + ecore, -14.61740392, rtol=0, atol=5e-9)
+
+
+@pytest.mark.parametrize("name,occs,energy", [("li2_ccpvdz", (3, 3), -14.878455349),
+                                              ("be_ccpvdz", (2, 2), -14.600556994)])
+def test_run_hci_doci(oracle, name, occs, energy):
+    """test_run_hci (test_routines.py:442-470) for DOCI: eps = 1e-4 iterations, then
+    eps = 0 until the space is complete; the energy is the DOCI energy."""
+    ecore, one, two = _load(name)
+    nb = one.shape[0]
+    dets = _hf("doci", occs[0], occs[1])
+    e, c = _solve(oracle, "doci", nb, *occs, dets, one, two)
+    niter = 0
+    for eps in (1e-4, 0.0):
+        added = 1
+        while added:
+            new = oracle.add_hci("doci", nb, *occs, dets, one, two, c, eps)
+            added = len(new)
+            if added:
+                dets = np.concatenate([dets, new])
+            e, c = _solve(oracle, "doci", nb, *occs, dets, one, two)
+            niter += eps > 0
+        if eps > 0:
+            assert niter > 1 and len(dets) <= comb(nb, occs[0], exact=True)
+    assert len(dets) == comb(nb, occs[0], exact=True)
+    npt.assert_allclose(e + ecore, energy, rtol=0, atol=2e-9)
+
+
+@pytest.mark.parametrize(
+    "name,kind,occs,energy",
+    [
+        ("be_ccpvdz", "fullci", (2, 2), -14.61740346031583),
+        ("be_ccpvdz", "doci", (2, 2), -14.61920612157886),
+        ("li2_ccpvdz", "doci", (3, 3), -14.90042952400693),
+        ("h2o_ccpvdz", "doci", (5, 5), -76.04227376501808),
+    ],
+)
+def test_enpt2_reference_values(oracle, name, kind, occs, energy):
+    """test_enpt2 (test_routines.py:473-491): excitations 0..nocc-2 from HF."""
+    ecore, one, two = _load(name)
+    nb = one.shape[0]
+    hf = _hf(kind, *occs)
+    dets = hf[:0]
+    for e in range(0, max(occs[0] + occs[1] - 1, 1)):
+        dets = oracle.add_excited_dets(kind, nb, *occs, dets, hf[0], e)
+    e0, c = _solve(oracle, kind, nb, *occs, dets, one, two)
+    e, _ = oracle.enpt2(kind, nb, *occs, dets, one, two, c, e0 + ecore, 1e-4, ecore)
+    npt.assert_allclose(e, energy, rtol=1e-12)
+
+
+def _within_rank(basis_all, ref_bits, e):
+    """Members of basis_all (spin-orbital bit strings) at excitation rank e from ref."""
+    return {int(b) for b in basis_all if bin(int(b) ^ ref_bits).count("1") == 2 * e}
+
+
+@pytest.mark.parametrize("kind,nb,nu,nd", [("fullci", 6, 3, 2), ("doci", 7, 3, 3), ("genci", 8, 3, 0)])
+@pytest.mark.parametrize("e", [0, 1, 2, 3])
+def test_add_excited_dets_set_brute_force(oracle, kind, nb, nu, nd, e):
+    """add_excited_dets(e, ref) adds exactly the determinants of excitation rank e
+    (of the given kind) not yet present, each once."""
+    if kind == "fullci":
+        ups = [sum(1 << p for p in c) for c in itertools.combinations(range(nb), nu)]
+        dns = [sum(1 << p for p in c) for c in itertools.combinations(range(nb), nd)]
+        allb = np.array([u | (d << nb) for u in ups for d in dns], dtype=np.int64)
+        ref = np.array([ups[3], dns[5]], dtype=np.uint64)
+        refb = int(ref[0]) | (int(ref[1]) << nb)
+        present = np.array([[ups[0], dns[0]], [ups[7], dns[2]]], dtype=np.uint64)
+        to_bits = lambda d: jw.fullci_basis(d, nb)
+    elif kind == "doci":
+        allb = np.array([sum(1 << p for p in c) for c in itertools.combinations(range(nb), nu)], dtype=np.int64)
+        allb = allb | (allb << nb)
+        ref = np.array([0b0101100], dtype=np.uint64)
+        refb = int(ref[0]) | (int(ref[0]) << nb)
+        present = np.array([0b0000111, 0b1110000], dtype=np.uint64)
+        to_bits = lambda d: jw.doci_basis(d, nb)
+    else:
+        allb = np.array([sum(1 << p for p in c) for c in itertools.combinations(range(nb), nu)], dtype=np.int64)
+        ref = np.array([0b10010010], dtype=np.uint64)
+        refb = int(ref[0])
+        present = np.array([0b00000111, 0b11100000], dtype=np.uint64)
+        to_bits = lambda d: np.asarray(d, dtype=np.uint64).reshape(-1).astype(np.int64)
+    out = oracle.add_excited_dets(kind, nb, nu, nd, present, ref.reshape(-1), e)
+    got = [int(b) for b in to_bits(out)]
+    assert len(got) == len(set(got))
+    pres = {int(b) for b in to_bits(present)}
+    want = _within_rank(allb, refb, e if kind != "doci" else 2 * e) - pres
+    assert [int(b) for b in to_bits(out[: len(present)])] == [int(b) for b in to_bits(present)]
+    assert set(got[len(present):]) == want
+
+
+@pytest.mark.parametrize("kind", ["doci", "genci"])
+def test_hci_criterion_brute_force(oracle, kind):
+    """add_hci adds exactly {J not in wfn : |H_JI| > eps/|c_I| for some I in wfn}
+    for DOCI (pair hops, |v_kl|, hci.cpp:22-47) and GenCI (hci.cpp:184-238), with
+    |H_JI| from the Jordan-Wigner Hamiltonian."""
+    if kind == "doci":
+        nb, no = 6, 3
+        _, one, two = seeded_integrals(nb, 6, 13)
+        alld = oracle.all_dets("doci", nb, no, no)
+        h, g = jw.spin_orbital_integrals(one, two)
+        H = jw.hamiltonian(h, g, jw.doci_basis(alld, nb))
+        ref = np.array([0b000111], dtype=np.uint64)
+        dets = oracle.add_excited_dets("doci", nb, no, no, ref, ref, 1)
+        nu, nd = no, no
+    else:
+        nb, nu, nd = 8, 3, 0
+        _, one, two = seeded_integrals(4, 6, 14)
+        one, two = spin_orbital(one, two)
+        alld = oracle.all_dets("genci", nb, nu, nd)
+        H = jw.hamiltonian(one, two, alld.astype(np.int64))
+        ref = np.array([0b00110001], dtype=np.uint64)
+        dets = oracle.add_excited_dets("genci", nb, nu, nd, ref, ref, 1)
+    idx = [int(np.where(alld == d)[0][0]) for d in dets]
+    c = np.random.default_rng(4).normal(size=len(dets))
+    for eps in (0.0, 1e-3, 2e-2):
+        new = oracle.add_hci(kind, nb, nu, nd, dets, one, two, c, eps)
+        got = {int(x) for x in new}
+        want = set()
+        for col, i in enumerate(idx):
+            for j in range(len(alld)):
+                if j not in idx and abs(H[j, i]) > eps / abs(c[col]):
+                    want.add(int(alld[j]))
+        assert got == want and len(new) == len(got)
+
+
+@pytest.mark.parametrize("kind", ["fullci", "genci"])
+@pytest.mark.parametrize("eps", [0.0, 5e-3])
+def test_enpt2_brute_force(oracle, kind, eps):
+    """ENPT2 from the dense Jordan-Wigner Hamiltonian: external determinants a
+    reached from some i with |H_ai| > eps/|c_i|, numerator sum over those i of
+    H_ai c_i, denominator e - H_aa."""
+    if kind == "fullci":
+        nb, nu, nd = 5, 2, 2
+        _, one, two = seeded_integrals(nb, 6, 15)
+        alld = oracle.all_dets("fullci", nb, nu, nd)
+        h, g = jw.spin_orbital_integrals(one, two)
+        H = jw.hamiltonian(h, g, jw.fullci_basis(alld, nb))
+        ref = np.array([[0b00011, 0b00011]], dtype=np.uint64)
+        dets = oracle.add_excited_dets("fullci", nb, nu, nd, ref, ref[0], 1)
+        find = lambda d: int(np.where((alld == d).all(1))[0][0])
+    else:
+        nb, nu, nd = 8, 3, 0
+        _, one, two = seeded_integrals(4, 6, 16)
+        one, two = spin_orbital(one, two)
+        alld = oracle.all_dets("genci", nb, nu, nd)
+        H = jw.hamiltonian(one, two, alld.astype(np.int64))
+        ref = np.array([0b00110001], dtype=np.uint64)
+        dets = oracle.add_excited_dets("genci", nb, nu, nd, ref, ref, 1)
+        find = lambda d: int(np.where(alld == d)[0][0])
+    idx = [find(d) for d in dets]
+    c = np.random.default_rng(6).normal(size=len(dets))
+    energy = -4.0
+    want, nt_want = 0.0, 0
+    for a in range(len(alld)):
+        if a in idx:
+            continue
+        num, hit = 0.0, False
+        for col, i in enumerate(idx):
+            if H[a, i] != 0.0 and abs(H[a, i]) > eps / abs(c[col]):
+                num += H[a, i] * c[col]
+                hit = True
+        if hit:
+            want += num * num / (energy - H[a, a])
+            nt_want += 1
+    e, nt = oracle.enpt2(kind, nb, nu, nd, dets, one, two, c, energy, eps)
+    assert nt == nt_want
+    npt.assert_allclose(e - energy, want, rtol=1e-11)
