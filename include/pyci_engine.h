@@ -153,8 +153,11 @@ int pyci_op_bench_sigma_parts(pyci_op *op, int64_t nvec, int64_t nrep, int fp32,
 /* solve(n, c0, ncv, maxiter, tol) (sparseop.cpp:114-146,161-173): lowest n
  * eigenpairs by block Davidson on the GPU. evals (n, ecore added) and evecs
  * (n, nrow) are returned in the reference order: descending eigenvalue when
- * n > 1. c0 (nrow) or NULL. ncv/maxiter < 0 select the reference defaults
- * ncv = min(nrow, max(2n+1, 20)) and maxiter = n*nrow*10. */
+ * n > 1. c0 (nrow) or NULL. ncv/maxiter <= 0 select this library's defaults,
+ * which differ from the reference's Lanczos ones (ncv = min(nrow, max(2n+1, 20)),
+ * maxiter = n*nrow*10): the Davidson subspace ncv = min(nrow, max(14, 2n+1, 6n))
+ * (thick restarts keep 2n Ritz vectors) and maxiter = max(1000, min(n*nrow*10,
+ * 100000)) iterations. An explicit ncv is raised to at least 2n+1. */
 int pyci_op_solve(pyci_op *op, int64_t n, const double *c0, int64_t ncv, int64_t maxiter, double tol, int precision,
                   double *evals, double *evecs, pyci_solve_stats *stats);
 void pyci_op_destroy(pyci_op *op);
@@ -183,6 +186,16 @@ int pyci_compute_enpt2(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs, doubl
  *           <a+_p a_q>, <a+_p a+_q a_s a_r>; the reference routine at
  *           rdm.cpp:532-632 has index errors, so it is not reproduced). */
 int pyci_compute_rdms(pyci_wfn *wfn, const double *coeffs, double *rdm1, double *rdm2);
+
+/* compute_transition_rdms (rdm.cpp:634-899): for every det I of wfn1 and every
+ * excitation J of I present in wfn2 (and I itself), c1_I c2_J times the phase
+ * into [p,q] = <1| a+_p a_q |2> and [p,q,r,s] = <1| a+_p a+_q a_s a_r |2>.
+ * Shapes as pyci_compute_rdms. Both sets must have the same kind, nbasis and
+ * occupations. GenCI is the exact operator algebra (the reference routine,
+ * rdm.cpp:901-1009, reads coeffs2 at the wrong index and writes past its
+ * arrays). coeffs1/coeffs2: host float64, one per determinant. */
+int pyci_compute_transition_rdms(pyci_wfn *wfn1, pyci_wfn *wfn2, const double *coeffs1, const double *coeffs2,
+                                 double *rdm1, double *rdm2);
 
 /* ------------------------------------------------------------------------
  * Learned selector. No counterpart in the reference: it proposes which
@@ -225,6 +238,38 @@ int pyci_mlp_train(pyci_mlp *m, const pyci_wfn *wfn, const double *coeffs, doubl
  * the first-occurrence order. nadded may be NULL. */
 int pyci_add_hci_mlp(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs, double eps, pyci_mlp *m, double pool,
                      int64_t nadd, int score_mode, int64_t *nadded);
+
+/* ------------------------------------------------------------------------
+ * FanCI geminal objectives. Replace AP1roGObjective / APIGObjective
+ * (fanci.cpp:21-217, ap1rog.cpp:21-220, apig.cpp:21-195): the projected
+ * Schrodinger equations f_n = <n|H|Psi> - E <n|Psi> over the P space, their
+ * Jacobian, and the geminal overlaps (permanents) over the S space.
+ * ------------------------------------------------------------------------ */
+#define PYCI_FANCI_AP1ROG 0 /* parameters c[i][a]: nocc x (nbasis - nocc), reference = first nocc orbitals */
+#define PYCI_FANCI_APIG 1   /* parameters c[p][i]: nbasis x nocc */
+typedef struct pyci_fanci pyci_fanci;
+
+/* op: a non-symmetric CSR operator with nproj rows and nconn columns over the
+ * DOCI wave function wfn (its first nconn determinants are the S space, the
+ * first nproj the P space); the operator must outlive the objective. Optional
+ * constraints (counts may be 0, pointers then NULL): determinant overlaps
+ * c[idx_det[i]] - val_det[i] and parameters x[idx_param[i]] - val_param[i]. */
+int pyci_fanci_create(pyci_op *op, pyci_wfn *wfn, int kind, int64_t ndet_cons, const int64_t *idx_det,
+                      const double *val_det, int64_t nparam_cons, const int64_t *idx_param, const double *val_param,
+                      pyci_fanci **out);
+void pyci_fanci_destroy(pyci_fanci *f);
+/* nparam excludes the energy; nequation = nproj + constraints. */
+int pyci_fanci_info(const pyci_fanci *f, int64_t *nparam, int64_t *nproj, int64_t *nconn, int64_t *nequation);
+/* overlap (Objective::py_overlap, fanci.cpp:226-231): y[nconn] from x[nparam]. */
+int pyci_fanci_overlap(pyci_fanci *f, const double *x, double *y);
+/* d_overlap (fanci.cpp:233-239): y column-major nconn x nparam. */
+int pyci_fanci_d_overlap(pyci_fanci *f, const double *x, double *y);
+/* objective (fanci.cpp:129-160): x[nparam + 1] (energy last) -> y[nequation]. */
+int pyci_fanci_objective(pyci_fanci *f, const double *x, double *y);
+/* jacobian (fanci.cpp:162-217): y column-major nequation x (nparam + 1); the
+ * parameter columns come from one product of the operator with all derivative
+ * columns instead of one perform_op per parameter (fanci.cpp:197-204). */
+int pyci_fanci_jacobian(pyci_fanci *f, const double *x, double *y);
 
 /* ------------------------------------------------------------------------
  * Batched solves of many uneven FullCI problems. No counterpart in the

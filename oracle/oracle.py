@@ -69,6 +69,10 @@ def lib():
         L.oracle_enpt2.restype = ctypes.c_double
         L.oracle_enpt2.argtypes = [ctypes.c_int, i64, i64, i64, i64, u64p, dp, dp, dp, ctypes.c_double,
                                    ctypes.c_double, i64p]
+        for nm in ("oracle_trdms_doci", "oracle_trdms_fullci", "oracle_trdms_genci"):
+            f = getattr(L, nm)
+            f.restype = None
+            f.argtypes = [i64, i64, u64p, i64, u64p, dp, dp, dp, dp]
         L.oracle_diagonal.restype = None
         L.oracle_diagonal.argtypes = [ctypes.c_int, i64, i64, i64, i64, u64p, dp, dp, dp]
         _lib = L
@@ -126,9 +130,16 @@ def add_excited_dets(kind, nbasis, nup, ndn, dets, ref, exc):
     ref, rp = _u64(np.asarray(ref).reshape(-1))
     k = nkey(kind)
     n0 = dets.size // k
-    cap = n0 + binomial(nbasis, nup) * (binomial(nbasis, ndn) if kind == "fullci" else 1) + 1
+    # at most the number of rank-exc excitations of ref is added
+    if kind == "fullci":
+        nexc = sum(binomial(nup, a) * binomial(nbasis - nup, a) * binomial(ndn, exc - a) * binomial(nbasis - ndn, exc - a)
+                   for a in range(exc + 1) if a <= nup and exc - a <= ndn)
+    else:
+        nexc = binomial(nup, exc) * binomial(nbasis - nup, exc)
+    cap = n0 + nexc + 1
     out = np.zeros(cap * k, dtype=np.uint64)
     n = lib().oracle_add_excited_dets(KIND[kind], nbasis, nup, ndn, n0, dp, rp, exc, _u64(out)[1], cap)
+    assert n <= cap
     out = out[: n * k]
     return out.reshape(n, 2) if kind == "fullci" else out
 
@@ -184,17 +195,22 @@ def matvec(csr, x):
     return y
 
 
-def add_hci(kind, nbasis, nup, ndn, dets, one_mo, two_mo, coeffs, eps):
-    """add_hci restated serially; new dets in first-occurrence order (hci.cpp:49-279)."""
+def add_hci(kind, nbasis, nup, ndn, dets, one_mo, two_mo, coeffs, eps, cap=None):
+    """add_hci restated serially; new dets in first-occurrence order (hci.cpp:49-279).
+    `cap` bounds the number of new determinants (default: the full space, at most
+    2^25); more raises."""
     dets, dp = _u64(np.asarray(dets).reshape(-1))
     k = nkey(kind)
     ndet = dets.size // k
-    cap = binomial(nbasis, nup) * (binomial(nbasis, ndn) if kind == "fullci" else 1) + 1
+    if cap is None:
+        cap = min(binomial(nbasis, nup) * (binomial(nbasis, ndn) if kind == "fullci" else 1), 1 << 25) + 1
     out = np.zeros(cap * k, dtype=np.uint64)
     one_mo, op1 = _f64(one_mo)
     two_mo, op2 = _f64(two_mo)
     coeffs, cp = _f64(coeffs)
     n = lib().oracle_add_hci(KIND[kind], nbasis, nup, ndn, ndet, dp, op1, op2, cp, float(eps), _u64(out)[1], cap)
+    if n > cap:
+        raise ValueError(f"add_hci appends {n} determinants, more than cap = {cap}")
     out = out[: n * k]
     return out.reshape(n, 2) if kind == "fullci" else out
 
@@ -216,6 +232,24 @@ def rdms(kind, nbasis, dets, coeffs):
     r1, r2 = np.zeros((n, n)), np.zeros((n, n, n, n))
     L.oracle_rdms_genci(n, dets.size, dp, cp, _f64(r1)[1], _f64(r2)[1])
     return r1, r2
+
+
+def transition_rdms(kind, nbasis, dets1, dets2, c1, c2):
+    """compute_transition_rdms: DOCI (d0, d2) and FullCI ((2,n,n), (3,n,n,n,n)) as
+    rdm.cpp:634-899; GenCI ((n,n), (n,n,n,n)) by the operator algebra."""
+    d1, p1 = _u64(np.asarray(dets1).reshape(-1))
+    d2, p2 = _u64(np.asarray(dets2).reshape(-1))
+    c1, q1 = _f64(c1)
+    c2, q2 = _f64(c2)
+    n, k = nbasis, nkey(kind)
+    if kind == "doci":
+        a, b = np.zeros((n, n)), np.zeros((n, n))
+    elif kind == "fullci":
+        a, b = np.zeros((2, n, n)), np.zeros((3, n, n, n, n))
+    else:
+        a, b = np.zeros((n, n)), np.zeros((n, n, n, n))
+    getattr(lib(), "oracle_trdms_" + kind)(n, d1.size // k, p1, d2.size // k, p2, q1, q2, _f64(a)[1], _f64(b)[1])
+    return a, b
 
 
 def diagonal(kind, nbasis, nup, ndn, dets, one_mo, two_mo):

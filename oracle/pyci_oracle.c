@@ -1233,3 +1233,227 @@ double oracle_enpt2(int kind, i64 nbasis, i64 nup, i64 ndn, i64 ndet, const u64 
     dset_free(&s);
     return corr;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Transition RDMs (rdm.cpp:634-899; GenCI by correct math)                  */
+/* ------------------------------------------------------------------------ */
+
+/* compute_transition_rdms(DOCIWfn) (rdm.cpp:634-672): for every det I of set 1
+ * and its pair hops J (k -> l) present in set 2, d0[k,l] += c1_I c2_J; the
+ * diagonal I itself (if in set 2) adds c1_I c2_I' to d0[k,k] and d2[k,l], d2[l,k]
+ * over its occupied pairs. */
+void oracle_trdms_doci(i64 nbasis, i64 ndet1, const u64 *dets1, i64 ndet2, const u64 *dets2, const double *c1,
+                       const double *c2, double *d0, double *d2) {
+    dset s;
+    dset_from(&s, 1, ndet2, dets2);
+    i64 n = nbasis, occs[64], virs[64];
+    memset(d0, 0, sizeof(double) * n * n);
+    memset(d2, 0, sizeof(double) * n * n);
+    for (i64 idet = 0; idet < ndet1; ++idet) {
+        u64 det = dets1[idet];
+        int no = occs_of(det, occs), nv = virs_of(det, n, virs);
+        i64 jd = dset_find(&s, &det);
+        double val1 = (jd != -1) ? c1[idet] * c2[jd] : 0.0;
+        for (int i = 0; i < no; ++i) {
+            i64 k = occs[i];
+            d0[k * (n + 1)] += val1;
+            for (int j = i + 1; j < no; ++j) {
+                i64 l = occs[j];
+                d2[n * k + l] += val1;
+                d2[n * l + k] += val1;
+            }
+            for (int j = 0; j < nv; ++j) {
+                i64 l = virs[j];
+                u64 e = (det & ~(1ULL << k)) | (1ULL << l);
+                jd = dset_find(&s, &e);
+                if (jd != -1) d0[n * k + l] += c1[idet] * c2[jd];
+            }
+        }
+    }
+    dset_free(&s);
+}
+
+/* compute_transition_rdms(FullCIWfn) (rdm.cpp:674-899): rdm1 = (aa, bb), rdm2 =
+ * (aaaa, bbbb, abab); element [p,q] = <1| a+_p a_q |2>, [p,q,r,s] =
+ * <1| a+_p a+_q a_s a_r |2>, one term per (I in set 1, J in set 2) with the
+ * reference's index images. */
+void oracle_trdms_fullci(i64 nbasis, i64 ndet1, const u64 *dets1, i64 ndet2, const u64 *dets2, const double *c1,
+                         const double *c2, double *rdm1, double *rdm2) {
+    dset s;
+    dset_from(&s, 2, ndet2, dets2);
+    i64 n1 = nbasis, n2 = n1 * n1, n3 = n2 * n1, n4 = n2 * n2;
+    double *aa = rdm1, *bb = rdm1 + n2, *aaaa = rdm2, *bbbb = rdm2 + n4, *abab = rdm2 + 2 * n4;
+    memset(rdm1, 0, sizeof(double) * 2 * n2);
+    memset(rdm2, 0, sizeof(double) * 3 * n4);
+    i64 ou[64], od[64], vu[64], vd[64];
+#define A4(X, p, q, r, t) X[(p) * n3 + (q) * n2 + (r) * n1 + (t)]
+    for (i64 idet = 0; idet < ndet1; ++idet) {
+        u64 up = dets1[2 * idet], dn = dets1[2 * idet + 1];
+        int nou = occs_of(up, ou), nod = occs_of(dn, od), nvu = virs_of(up, n1, vu), nvd = virs_of(dn, n1, vd);
+        u64 key[2] = {up, dn};
+        i64 jd = dset_find(&s, key);
+        const double ci = c1[idet];
+        double val1 = (jd != -1) ? ci * c2[jd] : 0.0;
+        for (int i = 0; i < nou; ++i) {
+            i64 ii = ou[i];
+            aa[(n1 + 1) * ii] += val1;
+            for (int k = i + 1; k < nou; ++k) {
+                i64 kk = ou[k];
+                A4(aaaa, ii, kk, ii, kk) += val1;
+                A4(aaaa, ii, kk, kk, ii) -= val1;
+                A4(aaaa, kk, ii, ii, kk) -= val1;
+                A4(aaaa, kk, ii, kk, ii) += val1;
+            }
+            for (int k = 0; k < nod; ++k) A4(abab, ii, od[k], ii, od[k]) += val1;
+            for (int j = 0; j < nvu; ++j) {
+                i64 jj = vu[j];
+                u64 eu = (up & ~(1ULL << ii)) | (1ULL << jj);
+                i64 sign_up = oracle_phase_single(up, ii, jj);
+                u64 k1[2] = {eu, dn};
+                jd = dset_find(&s, k1);
+                if (jd != -1) {
+                    double v = ci * c2[jd] * sign_up;
+                    aa[ii * n1 + jj] += v;
+                    for (int k = 0; k < nou; ++k) {
+                        if (k == i) continue;
+                        i64 kk = ou[k];
+                        A4(aaaa, ii, kk, jj, kk) += v;
+                        A4(aaaa, ii, kk, kk, jj) -= v;
+                        A4(aaaa, kk, ii, kk, jj) += v;
+                        A4(aaaa, kk, ii, jj, kk) -= v;
+                    }
+                    for (int k = 0; k < nod; ++k) A4(abab, ii, od[k], jj, od[k]) += v;
+                }
+                for (int k = 0; k < nod; ++k)
+                    for (int l = 0; l < nvd; ++l) {
+                        i64 kk = od[k], ll = vd[l];
+                        u64 k2[2] = {eu, (dn & ~(1ULL << kk)) | (1ULL << ll)};
+                        jd = dset_find(&s, k2);
+                        if (jd != -1) A4(abab, ii, kk, jj, ll) += ci * c2[jd] * sign_up * oracle_phase_single(dn, kk, ll);
+                    }
+                for (int k = i + 1; k < nou; ++k)
+                    for (int l = j + 1; l < nvu; ++l) {
+                        i64 kk = ou[k], ll = vu[l];
+                        u64 k2[2] = {(eu & ~(1ULL << kk)) | (1ULL << ll), dn};
+                        jd = dset_find(&s, k2);
+                        if (jd != -1) {
+                            double v = ci * c2[jd] * oracle_phase_double(up, ii, kk, jj, ll);
+                            A4(aaaa, ii, kk, jj, ll) += v;
+                            A4(aaaa, ii, kk, ll, jj) -= v;
+                            A4(aaaa, kk, ii, jj, ll) -= v;
+                            A4(aaaa, kk, ii, ll, jj) += v;
+                        }
+                    }
+            }
+        }
+        for (int i = 0; i < nod; ++i) {
+            i64 ii = od[i];
+            bb[(n1 + 1) * ii] += val1;
+            for (int k = i + 1; k < nod; ++k) {
+                i64 kk = od[k];
+                A4(bbbb, ii, kk, ii, kk) += val1;
+                A4(bbbb, ii, kk, kk, ii) -= val1;
+                A4(bbbb, kk, ii, ii, kk) -= val1;
+                A4(bbbb, kk, ii, kk, ii) += val1;
+            }
+            for (int j = 0; j < nvd; ++j) {
+                i64 jj = vd[j];
+                u64 ed = (dn & ~(1ULL << ii)) | (1ULL << jj);
+                u64 k1[2] = {up, ed};
+                jd = dset_find(&s, k1);
+                if (jd != -1) {
+                    double v = ci * c2[jd] * oracle_phase_single(dn, ii, jj);
+                    bb[ii * n1 + jj] += v;
+                    for (int k = 0; k < nou; ++k) A4(abab, ou[k], ii, ou[k], jj) += v;
+                    for (int k = 0; k < nod; ++k) {
+                        if (k == i) continue;
+                        i64 kk = od[k];
+                        A4(bbbb, ii, kk, jj, kk) += v;
+                        A4(bbbb, ii, kk, kk, jj) -= v;
+                        A4(bbbb, kk, ii, kk, jj) += v;
+                        A4(bbbb, kk, ii, jj, kk) -= v;
+                    }
+                }
+                for (int k = i + 1; k < nod; ++k)
+                    for (int l = j + 1; l < nvd; ++l) {
+                        i64 kk = od[k], ll = vd[l];
+                        u64 k2[2] = {up, (ed & ~(1ULL << kk)) | (1ULL << ll)};
+                        jd = dset_find(&s, k2);
+                        if (jd != -1) {
+                            double v = ci * c2[jd] * oracle_phase_double(dn, ii, kk, jj, ll);
+                            A4(bbbb, ii, kk, jj, ll) += v;
+                            A4(bbbb, ii, kk, ll, jj) -= v;
+                            A4(bbbb, kk, ii, jj, ll) -= v;
+                            A4(bbbb, kk, ii, ll, jj) += v;
+                        }
+                    }
+            }
+        }
+    }
+#undef A4
+    dset_free(&s);
+}
+
+/* Transition RDMs of two GenCI sets by the operator algebra (the reference
+ * routine, rdm.cpp:901-1009, reads coeffs2 at the wrong index and writes past
+ * its arrays): rdm1[p,q] = <1| a+_p a_q |2>, rdm2[p,q,r,s] = <1| a+_p a+_q a_s a_r |2>. */
+void oracle_trdms_genci(i64 nbasis, i64 ndet1, const u64 *dets1, i64 ndet2, const u64 *dets2, const double *c1,
+                        const double *c2, double *rdm1, double *rdm2) {
+    dset s;
+    dset_from(&s, 1, ndet2, dets2);
+    i64 n1 = nbasis, n2 = n1 * n1, n3 = n2 * n1, n4 = n2 * n2;
+    memset(rdm1, 0, sizeof(double) * n2);
+    memset(rdm2, 0, sizeof(double) * n4);
+    i64 occs[64], virs[64];
+#define A4(p, q, r, t) rdm2[(p) * n3 + (q) * n2 + (r) * n1 + (t)]
+    for (i64 idet = 0; idet < ndet1; ++idet) {
+        u64 det = dets1[idet];
+        int no = occs_of(det, occs), nv = virs_of(det, n1, virs);
+        i64 jd = dset_find(&s, &det);
+        const double ci = c1[idet];
+        double val1 = (jd != -1) ? ci * c2[jd] : 0.0;
+        for (int i = 0; i < no; ++i) {
+            i64 ii = occs[i];
+            rdm1[(n1 + 1) * ii] += val1;
+            for (int k = i + 1; k < no; ++k) {
+                i64 kk = occs[k];
+                A4(ii, kk, ii, kk) += val1;
+                A4(ii, kk, kk, ii) -= val1;
+                A4(kk, ii, ii, kk) -= val1;
+                A4(kk, ii, kk, ii) += val1;
+            }
+            for (int j = 0; j < nv; ++j) {
+                i64 jj = virs[j];
+                u64 e = (det & ~(1ULL << ii)) | (1ULL << jj);
+                jd = dset_find(&s, &e);
+                if (jd != -1) {
+                    double v = ci * c2[jd] * oracle_phase_single(det, ii, jj);
+                    rdm1[ii * n1 + jj] += v;
+                    for (int k = 0; k < no; ++k) {
+                        if (k == i) continue;
+                        i64 kk = occs[k];
+                        A4(ii, kk, jj, kk) += v;
+                        A4(ii, kk, kk, jj) -= v;
+                        A4(kk, ii, kk, jj) += v;
+                        A4(kk, ii, jj, kk) -= v;
+                    }
+                }
+                for (int k = i + 1; k < no; ++k)
+                    for (int l = j + 1; l < nv; ++l) {
+                        i64 kk = occs[k], ll = virs[l];
+                        u64 e2 = (e & ~(1ULL << kk)) | (1ULL << ll);
+                        jd = dset_find(&s, &e2);
+                        if (jd != -1) {
+                            double v = ci * c2[jd] * oracle_phase_double(det, ii, kk, jj, ll);
+                            A4(ii, kk, jj, ll) += v;
+                            A4(ii, kk, ll, jj) -= v;
+                            A4(kk, ii, jj, ll) -= v;
+                            A4(kk, ii, ll, jj) += v;
+                        }
+                    }
+            }
+        }
+    }
+#undef A4
+    dset_free(&s);
+}

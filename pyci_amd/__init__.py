@@ -12,7 +12,7 @@ from pyci_amd._pyci import __version__, c_long, c_ulong, c_double
 from pyci_amd._pyci import secondquant_op, wavefunction, one_spin_wfn, two_spin_wfn
 from pyci_amd._pyci import doci_wfn, fullci_wfn, genci_wfn, sparse_op
 from pyci_amd._pyci import get_num_threads, set_num_threads, popcnt, ctz
-from pyci_amd._pyci import compute_rdms, compute_enpt2, add_hci, device_count, mlp_selector, MLP_NPARAM, batch_solve
+from pyci_amd._pyci import compute_rdms, compute_transition_rdms, compute_enpt2, add_hci, device_count, mlp_selector, MLP_NPARAM, batch_solve
 
 from pyci_amd.utility import make_senzero_integrals, reduce_senzero_integrals, spinize_rdms
 from pyci_amd.excitation_ci import add_excitations
@@ -40,6 +40,7 @@ __all__ = [
     "add_hci",
     "compute_rdms",
     "compute_enpt2",
+    "compute_transition_rdms",
     "make_senzero_integrals",
     "reduce_senzero_integrals",
     "spinize_rdms",

@@ -99,7 +99,12 @@ struct Wfn {
     i64 ndet = 0, nword = 0, nwdet = 0;  // nwdet: words per determinant (nword or 2*nword)
     i64 maxrank_up = 0, maxrank_dn = 0;
     int kind = PYCI_WFN_DOCI;
-    std::vector<u64> dets;
+    // Host copy of the determinants: dets_ holds the first host_ndet of the ndet.
+    // Determinants that add_hci appended on the device are copied here only when
+    // host code first reads them (hdets(), det_ptr(), lookups), so that an
+    // add_hci -> update -> solve loop keeps them on the device.
+    mutable std::vector<u64> dets_;
+    mutable i64 host_ndet = 0;
     mutable HostIndex index;
     mutable bool index_valid = true;
     bool full_colex = false;  // dets are exactly the full space in colex order
@@ -113,9 +118,16 @@ struct Wfn {
     void init(i64 nb, i64 nu, i64 nd);
 
     i64 length() const { return ndet; }
-    void squeeze() { dets.shrink_to_fit(); }
-    void reserve(i64 n) { dets.reserve((size_t)(n * nwdet)); }
-    const u64 *det_ptr(i64 i) const { return &dets[(size_t)(i * nwdet)]; }
+    void squeeze() { dets_.shrink_to_fit(); }
+    void reserve(i64 n) { dets_.reserve((size_t)(n * nwdet)); }
+    void sync_host() const;              // copies device-only determinants to dets_
+    const std::vector<u64> &hdets() const {
+        sync_host();
+        return dets_;
+    }
+    const u64 *det_ptr(i64 i) const { return &hdets()[(size_t)(i * nwdet)]; }
+    i64 host_pending() const { return ndet - host_ndet; }  // determinants held only on the device
+    void set_dets(std::vector<u64> &&buf, i64 n);  // replaces the set (host and device mirror)
 
     void ensure_index() const;
     i64 index_det(const u64 *d) const;

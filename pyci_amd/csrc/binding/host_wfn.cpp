@@ -24,11 +24,12 @@ void Wfn::init(i64 nb, i64 nu, i64 nd) {
     nvir_up = nb - nu;
     nvir_dn = nb - nd;
     ndet = 0;
+    host_ndet = 0;
     nword = nword_det(nb);
     nwdet = (kind == PYCI_WFN_FULLCI) ? 2 * nword : nword;
     maxrank_up = binomial(nb, nu);
     maxrank_dn = binomial(nb, nd);
-    dets.clear();
+    dets_.clear();
     index.clear();
     index_valid = true;
     full_colex = false;
@@ -36,39 +37,64 @@ void Wfn::init(i64 nb, i64 nu, i64 nd) {
     dev_ndet = 0;
 }
 
+// Copies the determinants that exist only on the device (appended there by
+// add_hci) to the host copy and its index.
+void Wfn::sync_host() const {
+    if (host_ndet >= ndet) return;
+    const i64 n = ndet - host_ndet;
+    const size_t at = dets_.size();
+    dets_.resize(at + (size_t)(n * nwdet));
+    check(pyci_wfn_get_dets(dev.get(), host_ndet, ndet, dets_.data() + at));
+    if (index_valid)
+        for (i64 i = host_ndet; i < ndet; ++i) index.insert(dets_, nwdet, i);
+    host_ndet = ndet;
+}
+
+void Wfn::set_dets(std::vector<u64> &&buf, i64 n) {
+    reset();
+    dets_.swap(buf);
+    ndet = host_ndet = n;
+    index_valid = false;  // built only when a host lookup needs it
+}
+
 void Wfn::ensure_index() const {
+    sync_host();
     if (!index_valid) {
-        index.rebuild(dets, nwdet, ndet);
+        index.rebuild(dets_, nwdet, ndet);
         index_valid = true;
     }
 }
 
 i64 Wfn::index_det(const u64 *d) const {
     ensure_index();
-    return index.find(dets, nwdet, d);
+    return index.find(dets_, nwdet, d);
 }
 
 i64 Wfn::add_det(const u64 *d) {
     if (index_det(d) >= 0) return -1;
-    dets.insert(dets.end(), d, d + nwdet);
-    index.insert(dets, nwdet, ndet);
+    dets_.insert(dets_.end(), d, d + nwdet);
+    index.insert(dets_, nwdet, ndet);
     full_colex = false;
+    host_ndet = ndet + 1;
     return ndet++;
 }
 
 void Wfn::append_unique(const u64 *d, i64 n) {
     if (n <= 0) return;
-    dets.insert(dets.end(), d, d + n * nwdet);
+    sync_host();
+    dets_.insert(dets_.end(), d, d + n * nwdet);
     if (index_valid) {
-        for (i64 i = 0; i < n; ++i) index.insert(dets, nwdet, ndet + i);
+        for (i64 i = 0; i < n; ++i) index.insert(dets_, nwdet, ndet + i);
     }
     ndet += n;
+    host_ndet = ndet;
     full_colex = false;
 }
 
 void Wfn::reset() {
-    dets.clear();
+    dets_.clear();
     ndet = 0;
+    host_ndet = 0;
     index.clear();
     index_valid = true;
     full_colex = false;
@@ -88,20 +114,21 @@ pyci_wfn *Wfn::device() {
     if (nbasis > 64) throw py::value_error("the GPU engine supports nbasis <= 64");
     pyci_ctx *ctx = context();
     pyci_wfn *w = nullptr;
-    check(pyci_wfn_create(ctx, kind, nbasis, nocc_up, nocc_dn, ndet, dets.data(), &w));
+    check(pyci_wfn_create(ctx, kind, nbasis, nocc_up, nocc_dn, ndet, hdets().data(), &w));
     dev = std::shared_ptr<pyci_wfn>(w, pyci_wfn_destroy);
     dev_ndet = ndet;
     return w;
 }
 
+// Determinants appended on the device (add_hci) join the set; the host copy
+// follows only when host code reads them (sync_host).
 void Wfn::adopt_device_appends(i64 old_ndet) {
     i64 nd = 0;
     check(pyci_wfn_ndet(dev.get(), &nd));
     if (nd <= old_ndet) return;
-    std::vector<u64> buf((size_t)((nd - old_ndet) * nwdet));
-    check(pyci_wfn_get_dets(dev.get(), old_ndet, nd, buf.data()));
-    append_unique(buf.data(), nd - old_ndet);
-    dev_ndet = ndet;
+    ndet = nd;
+    dev_ndet = nd;
+    full_colex = false;
 }
 
 // add_all_dets (onespinwfn.cpp:173-217, twospinwfn.cpp:223-245): the full space
@@ -113,10 +140,7 @@ void Wfn::fill_all_dets() {
     const i64 n = (kind == PYCI_WFN_FULLCI) ? maxrank_up * maxrank_dn : maxrank_up;
     std::vector<u64> buf((size_t)(n * nwdet), 0ULL);
     if (n > 0) check(pyci_fill_all_dets(context(), kind, nbasis, nocc_up, nocc_dn, buf.data()));
-    reset();
-    dets.swap(buf);
-    ndet = n;
-    index_valid = false;  // built only when a host lookup needs it
+    set_dets(std::move(buf), n);
     full_colex = true;
 }
 
@@ -126,7 +150,7 @@ void Wfn::to_file(const std::string &filename) const {
     std::ofstream f(filename, std::ios::out | std::ios::binary);
     i64 hdr[4] = {ndet, nbasis, nocc_up, nocc_dn};
     bool ok = (bool)f.write((const char *)hdr, sizeof(hdr));
-    if (ok && ndet) ok = (bool)f.write((const char *)dets.data(), (std::streamsize)(sizeof(u64) * ndet * nwdet));
+    if (ok && ndet) ok = (bool)f.write((const char *)hdets().data(), (std::streamsize)(sizeof(u64) * ndet * nwdet));
     f.close();
     if (!ok) throw std::ios_base::failure("error writing file");
 }
@@ -139,9 +163,7 @@ void Wfn::load_file(const std::string &filename) {
     std::vector<u64> buf((size_t)(hdr[0] * nwdet));
     if (hdr[0] && !f.read((char *)buf.data(), (std::streamsize)(sizeof(u64) * buf.size())))
         throw std::ios_base::failure("error in file");
-    dets.swap(buf);
-    ndet = hdr[0];
-    index_valid = false;
+    set_dets(std::move(buf), hdr[0]);
 }
 
 std::pair<u64, u64> Wfn::rank_det(const u64 *d) const {

@@ -10,9 +10,8 @@ static i64 g_threads = 1;
 
 template <class W>
 static void copy_from(W &dst, const Wfn &src) {
-    dst.dets = src.dets;
-    dst.ndet = src.ndet;
-    dst.index_valid = false;
+    std::vector<u64> buf = src.hdets();
+    dst.set_dets(std::move(buf), src.ndet);
     dst.full_colex = src.full_colex;
 }
 
@@ -21,9 +20,7 @@ static std::unique_ptr<W> from_dets(i64 nb, i64 nu, i64 nd, Array<u64> a) {
     auto w = std::make_unique<W>(nb, nu, nd);
     i64 n = a.ndim() ? a.shape(0) : 0;
     if ((i64)a.size() != n * w->nwdet) throw py::value_error("determinant array has the wrong shape");
-    w->dets.assign(a.data(), a.data() + n * w->nwdet);
-    w->ndet = n;
-    w->index_valid = false;
+    w->set_dets(std::vector<u64>(a.data(), a.data() + n * w->nwdet), n);
     return w;
 }
 
@@ -34,15 +31,14 @@ static std::unique_ptr<W> from_occs(i64 nb, i64 nu, i64 nd, Array<i64> a) {
     const i64 per = two ? 2 * nu : nu;
     i64 n = a.ndim() ? a.shape(0) : 0;
     if ((i64)a.size() != n * per) throw py::value_error("occupation array has the wrong shape");
-    w->dets.assign((size_t)(n * w->nwdet), 0ULL);
+    std::vector<u64> buf((size_t)(n * w->nwdet), 0ULL);
     const i64 *p = a.data();
     for (i64 i = 0; i < n; ++i) {
-        u64 *d = &w->dets[(size_t)(i * w->nwdet)];
+        u64 *d = &buf[(size_t)(i * w->nwdet)];
         fill_det(nu, p + i * per, d);
         if (two) fill_det(nd, p + i * per + nu, d + w->nword);
     }
-    w->ndet = n;
-    w->index_valid = false;
+    w->set_dets(std::move(buf), n);
     return w;
 }
 
@@ -56,13 +52,12 @@ static std::unique_ptr<W> from_file(const std::string &fn) {
 // FullCI from DOCI (fullciwfn.cpp:22-33): alpha string = beta string.
 static std::unique_ptr<FullCIWfn> fullci_from_doci(const DOCIWfn &d) {
     auto w = std::make_unique<FullCIWfn>(d.nbasis, d.nocc_up, d.nocc_dn);
-    w->dets.resize((size_t)(d.ndet * w->nwdet));
+    std::vector<u64> buf((size_t)(d.ndet * w->nwdet));
     for (i64 i = 0; i < d.ndet; ++i) {
-        std::memcpy(&w->dets[(size_t)(i * w->nwdet)], d.det_ptr(i), sizeof(u64) * d.nword);
-        std::memcpy(&w->dets[(size_t)(i * w->nwdet + d.nword)], d.det_ptr(i), sizeof(u64) * d.nword);
+        std::memcpy(&buf[(size_t)(i * w->nwdet)], d.det_ptr(i), sizeof(u64) * d.nword);
+        std::memcpy(&buf[(size_t)(i * w->nwdet + d.nword)], d.det_ptr(i), sizeof(u64) * d.nword);
     }
-    w->ndet = d.ndet;
-    w->index_valid = false;
+    w->set_dets(std::move(buf), d.ndet);
     return w;
 }
 
@@ -71,16 +66,15 @@ static std::unique_ptr<FullCIWfn> fullci_from_doci(const DOCIWfn &d) {
 // (genciwfn.cpp:37); this conversion uses the beta string.
 static std::unique_ptr<GenCIWfn> genci_from_fullci(const FullCIWfn &f) {
     auto w = std::make_unique<GenCIWfn>(f.nbasis * 2, f.nocc, 0);
-    w->dets.assign((size_t)(f.ndet * w->nwdet), 0ULL);
+    std::vector<u64> buf((size_t)(f.ndet * w->nwdet), 0ULL);
     std::vector<i64> occ((size_t)f.nocc + 1);
     for (i64 i = 0; i < f.ndet; ++i) {
         fill_occs(f.nword, f.det_ptr(i), occ.data());
         fill_occs(f.nword, f.det_ptr(i) + f.nword, occ.data() + f.nocc_up);
         for (i64 j = 0; j < f.nocc_dn; ++j) occ[(size_t)(f.nocc_up + j)] += f.nbasis;
-        fill_det(f.nocc, occ.data(), &w->dets[(size_t)(i * w->nwdet)]);
+        fill_det(f.nocc, occ.data(), &buf[(size_t)(i * w->nwdet)]);
     }
-    w->ndet = f.ndet;
-    w->index_valid = false;
+    w->set_dets(std::move(buf), f.ndet);
     return w;
 }
 
@@ -160,7 +154,7 @@ struct MLPSelector {
         Array<float> y(w.ndet);
         if (w.ndet == 0) return y;
         py::gil_scoped_release nogil;
-        check(pyci_mlp_predict(net.get(), w.kind, w.nbasis, w.ndet, w.dets.data(), y.mutable_data()));
+        check(pyci_mlp_predict(net.get(), w.kind, w.nbasis, w.ndet, w.hdets().data(), y.mutable_data()));
         return y;
     }
     py::tuple loss_grad(const Wfn &w, Array<float> target) const {
@@ -169,7 +163,7 @@ struct MLPSelector {
         float loss = 0.0f;
         {
             py::gil_scoped_release nogil;
-            check(pyci_mlp_loss_grad(net.get(), w.kind, w.nbasis, w.ndet, w.dets.data(), target.data(), &loss,
+            check(pyci_mlp_loss_grad(net.get(), w.kind, w.nbasis, w.ndet, w.hdets().data(), target.data(), &loss,
                                      g.mutable_data()));
         }
         return py::make_tuple(loss, g);
@@ -241,6 +235,123 @@ static py::tuple rdms(Wfn &wfn, Array<double> coeffs) {
     return py::make_tuple(r1, r2);
 }
 
+static py::tuple trdms(Wfn &w1, Wfn &w2, Array<double> c1, Array<double> c2) {
+    if ((i64)c1.size() != w1.ndet || (i64)c2.size() != w2.ndet)
+        throw py::value_error("coeffs must have one entry per determinant");
+    if (w1.kind != w2.kind || w1.nbasis != w2.nbasis || w1.nocc_up != w2.nocc_up || w1.nocc_dn != w2.nocc_dn)
+        throw py::value_error("wave functions must have the same kind, nbasis and occupations");
+    const i64 n = w1.nbasis;
+    Array<double> r1, r2;
+    if (w1.kind == PYCI_WFN_DOCI) {
+        r1 = Array<double>({n, n});
+        r2 = Array<double>({n, n});
+    } else if (w1.kind == PYCI_WFN_FULLCI) {
+        r1 = Array<double>({(i64)2, n, n});
+        r2 = Array<double>({(i64)3, n, n, n, n});
+    } else {
+        r1 = Array<double>({n, n});
+        r2 = Array<double>({n, n, n, n});
+    }
+    pyci_wfn *d1 = w1.device();
+    pyci_wfn *d2 = w2.device();
+    {
+        py::gil_scoped_release nogil;
+        check(pyci_compute_transition_rdms(d1, d2, c1.data(), c2.data(), r1.mutable_data(), r2.mutable_data()));
+    }
+    return py::make_tuple(r1, r2);
+}
+
+// FanCI geminal objectives (AP1roGObjective / APIGObjective, fanci.cpp,
+// ap1rog.cpp, apig.cpp): overlaps, objective and Jacobian on the GPU.
+struct FanciObjective {
+    // declared before f so that it is destroyed after it: the objective's handle
+    // refers to the operator until pyci_fanci_destroy returns
+    std::shared_ptr<pyci_op> op;  // the operator the objective was built over (kept alive)
+    std::shared_ptr<pyci_fanci> f;
+    i64 nparam = 0, nproj = 0, nconn = 0, nequation = 0;
+
+    FanciObjective(int kind, SparseOp &o, DOCIWfn &w, py::object idc, py::object vdc, py::object ipc,
+                   py::object vpc) {
+        if (idc.is_none() != vdc.is_none() || ipc.is_none() != vpc.is_none())
+            throw py::value_error("constraint indices and values must be given together");
+        Array<i64> a_idc = idc.is_none() ? Array<i64>(0) : idc.cast<Array<i64>>();
+        Array<double> a_vdc = vdc.is_none() ? Array<double>(0) : vdc.cast<Array<double>>();
+        Array<i64> a_ipc = ipc.is_none() ? Array<i64>(0) : ipc.cast<Array<i64>>();
+        Array<double> a_vpc = vpc.is_none() ? Array<double>(0) : vpc.cast<Array<double>>();
+        if (a_idc.size() != a_vdc.size() || a_ipc.size() != a_vpc.size())
+            throw py::value_error("constraint indices and values differ in length");
+        pyci_fanci *h = nullptr;
+        check(pyci_fanci_create(o.op.get(), w.device(), kind, (i64)a_idc.size(), a_idc.size() ? a_idc.data() : nullptr,
+                                a_vdc.size() ? a_vdc.data() : nullptr, (i64)a_ipc.size(),
+                                a_ipc.size() ? a_ipc.data() : nullptr, a_vpc.size() ? a_vpc.data() : nullptr, &h));
+        f = std::shared_ptr<pyci_fanci>(h, pyci_fanci_destroy);
+        op = o.op;
+        int64_t np, nr, nc, ne;
+        check(pyci_fanci_info(h, &np, &nr, &nc, &ne));
+        nparam = np, nproj = nr, nconn = nc, nequation = ne;
+    }
+    void same_op(const SparseOp &o) const {
+        if (o.op.get() != op.get()) throw py::value_error("the objective was built over another operator");
+    }
+    void check_x(const Array<double> &x, i64 n) const {
+        if ((i64)x.size() < n) throw py::value_error("parameter array is too short");
+    }
+    Array<double> overlap(Array<double> x) const {
+        check_x(x, nparam);
+        Array<double> y(nconn);
+        py::gil_scoped_release nogil;
+        check(pyci_fanci_overlap(f.get(), x.data(), y.mutable_data()));
+        return y;
+    }
+    py::array_t<double, py::array::f_style> d_overlap(Array<double> x) const {
+        check_x(x, nparam);
+        py::array_t<double, py::array::f_style> y({nconn, nparam});
+        py::gil_scoped_release nogil;
+        check(pyci_fanci_d_overlap(f.get(), x.data(), y.mutable_data()));
+        return y;
+    }
+    Array<double> objective(const SparseOp &o, Array<double> x) const {
+        same_op(o);
+        check_x(x, nparam + 1);
+        Array<double> y(nequation);
+        py::gil_scoped_release nogil;
+        check(pyci_fanci_objective(f.get(), x.data(), y.mutable_data()));
+        return y;
+    }
+    py::array_t<double, py::array::f_style> jacobian(const SparseOp &o, Array<double> x) const {
+        same_op(o);
+        check_x(x, nparam + 1);
+        py::array_t<double, py::array::f_style> y({nequation, nparam + 1});
+        py::gil_scoped_release nogil;
+        check(pyci_fanci_jacobian(f.get(), x.data(), y.mutable_data()));
+        return y;
+    }
+};
+
+// one Python class per objective kind (pybind11 registers each C++ type once)
+template <int KIND>
+struct FanciKind : FanciObjective {
+    FanciKind(SparseOp &o, DOCIWfn &w, py::object a, py::object b, py::object c, py::object d)
+        : FanciObjective(KIND, o, w, a, b, c, d) {}
+};
+
+template <int KIND>
+static void bind_fanci(py::module_ &m, const char *name) {
+    using C = FanciKind<KIND>;
+    py::class_<C>(m, name)
+        .def(py::init<SparseOp &, DOCIWfn &, py::object, py::object, py::object, py::object>(), py::arg("op"),
+             py::arg("wfn"), py::arg("idx_det_cons") = py::none(), py::arg("det_cons") = py::none(),
+             py::arg("idx_param_cons") = py::none(), py::arg("param_cons") = py::none())
+        .def_readonly("nparam", &C::nparam)
+        .def_readonly("nproj", &C::nproj)
+        .def_readonly("nconn", &C::nconn)
+        .def_readonly("nequation", &C::nequation)
+        .def("overlap", &C::overlap, py::arg("x"))
+        .def("d_overlap", &C::d_overlap, py::arg("x"))
+        .def("objective", &C::objective, py::arg("op"), py::arg("x"))
+        .def("jacobian", &C::jacobian, py::arg("op"), py::arg("x"));
+}
+
 }  // namespace ph
 
 using namespace ph;
@@ -286,7 +397,10 @@ PYBIND11_MODULE(_pyci, m) {
         .def_readonly("nvir_up", &Wfn::nvir_up)
         .def_readonly("nvir_dn", &Wfn::nvir_dn)
         .def("__len__", &Wfn::length)
-        .def("squeeze", &Wfn::squeeze);
+        .def("squeeze", &Wfn::squeeze)
+        // determinants that add_hci appended on the device and host code has not
+        // read yet (they stay on the device across update and solve)
+        .def_property_readonly("device_only_dets", &Wfn::host_pending);
 
     auto common = [](auto &cls) {
         using C = typename std::remove_reference_t<decltype(cls)>::type;
@@ -416,6 +530,10 @@ PYBIND11_MODULE(_pyci, m) {
         .def("export", &SparseOp::export_csr)
         .def("diagonal", &SparseOp::diagonal);
 
+    // ---- FanCI geminal objectives ----
+    bind_fanci<PYCI_FANCI_AP1ROG>(m, "AP1roGObjective");
+    bind_fanci<PYCI_FANCI_APIG>(m, "APIGObjective");
+
     // ---- free functions ----
     m.def("get_num_threads", [] { return g_threads; });
     m.def("set_num_threads", [](i64 n) { g_threads = std::max<i64>(n, 1); }, py::arg("n"));
@@ -448,6 +566,15 @@ PYBIND11_MODULE(_pyci, m) {
           py::arg("eps") = 1.0e-5, py::arg("nthread") = -1);
     m.def("compute_enpt2", &enpt2<GenCIWfn>, py::arg("ham"), py::arg("wfn"), py::arg("coeffs"), py::arg("energy"),
           py::arg("eps") = 1.0e-5, py::arg("nthread") = -1);
+    m.def("compute_transition_rdms",
+          [](DOCIWfn &a, DOCIWfn &b, Array<double> c1, Array<double> c2) { return trdms(a, b, c1, c2); },
+          py::arg("wfn1"), py::arg("wfn2"), py::arg("coeffs1"), py::arg("coeffs2"));
+    m.def("compute_transition_rdms",
+          [](FullCIWfn &a, FullCIWfn &b, Array<double> c1, Array<double> c2) { return trdms(a, b, c1, c2); },
+          py::arg("wfn1"), py::arg("wfn2"), py::arg("coeffs1"), py::arg("coeffs2"));
+    m.def("compute_transition_rdms",
+          [](GenCIWfn &a, GenCIWfn &b, Array<double> c1, Array<double> c2) { return trdms(a, b, c1, c2); },
+          py::arg("wfn1"), py::arg("wfn2"), py::arg("coeffs1"), py::arg("coeffs2"));
     m.def("compute_rdms", [](DOCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));
     m.def("compute_rdms", [](FullCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));
     m.def("compute_rdms", [](GenCIWfn &w, Array<double> c) { return rdms(w, c); }, py::arg("wfn"), py::arg("coeffs"));

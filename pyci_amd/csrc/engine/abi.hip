@@ -555,6 +555,67 @@ int pyci_compute_rdms(pyci_wfn *wfn, const double *coeffs, double *rdm1, double 
     });
 }
 
+int pyci_compute_transition_rdms(pyci_wfn *wfn1, pyci_wfn *wfn2, const double *coeffs1, const double *coeffs2,
+                                 double *rdm1, double *rdm2) {
+    return guarded([&] {
+        PE_REQUIRE(wfn1 && wfn2 && rdm1 && rdm2, PYCI_EINVAL, "null argument");
+        set_device(wfn1->ctx);
+        trdms_device(wfn1, wfn2, coeffs1, coeffs2, rdm1, rdm2);
+    });
+}
+
+int pyci_fanci_create(pyci_op *op, pyci_wfn *wfn, int kind, int64_t ndet_cons, const int64_t *idx_det,
+                      const double *val_det, int64_t nparam_cons, const int64_t *idx_param, const double *val_param,
+                      pyci_fanci **out) {
+    return guarded([&] {
+        PE_REQUIRE(op && wfn && out, PYCI_EINVAL, "null argument");
+        PE_REQUIRE(ndet_cons >= 0 && nparam_cons >= 0, PYCI_EINVAL, "negative constraint count");
+        PE_REQUIRE((ndet_cons == 0 || (idx_det && val_det)) && (nparam_cons == 0 || (idx_param && val_param)),
+                   PYCI_EINVAL, "null constraint array");
+        PE_REQUIRE(op->ctx == wfn->ctx, PYCI_EINVAL, "operator and wave function belong to different contexts");
+        set_device(op->ctx);
+        *out = fanci_create(op, wfn, kind, ndet_cons, idx_det, val_det, nparam_cons, idx_param, val_param);
+    });
+}
+
+void pyci_fanci_destroy(pyci_fanci *f) {
+    if (!f) return;
+    (void)hipSetDevice(fanci_device(f));
+    fanci_destroy(f);
+}
+
+int pyci_fanci_info(const pyci_fanci *f, int64_t *nparam, int64_t *nproj, int64_t *nconn, int64_t *nequation) {
+    return guarded([&] { fanci_info(f, nparam, nproj, nconn, nequation); });
+}
+
+int pyci_fanci_overlap(pyci_fanci *f, const double *x, double *y) {
+    return guarded([&] {
+        PE_HIP(hipSetDevice(fanci_device(f)));
+        fanci_overlap(f, x, y);
+    });
+}
+
+int pyci_fanci_d_overlap(pyci_fanci *f, const double *x, double *y) {
+    return guarded([&] {
+        PE_HIP(hipSetDevice(fanci_device(f)));
+        fanci_d_overlap(f, x, y);
+    });
+}
+
+int pyci_fanci_objective(pyci_fanci *f, const double *x, double *y) {
+    return guarded([&] {
+        PE_HIP(hipSetDevice(fanci_device(f)));
+        fanci_objective(f, x, y);
+    });
+}
+
+int pyci_fanci_jacobian(pyci_fanci *f, const double *x, double *y) {
+    return guarded([&] {
+        PE_HIP(hipSetDevice(fanci_device(f)));
+        fanci_jacobian(f, x, y);
+    });
+}
+
 int pyci_batch_solve(pyci_ctx *ctx, int64_t nprob, const int64_t *nbasis, const int64_t *nocc_up,
                      const int64_t *nocc_dn, const double *ecore, const double *const *one_mo,
                      const double *const *two_mo, int64_t nroot, double tol, int precision, int64_t maxiter,

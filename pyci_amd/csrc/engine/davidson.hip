@@ -325,12 +325,15 @@ struct RitzCoef {
 
 template <int M>
 __global__ void __launch_bounds__(DT) k_ritz1(i64 nd, const double *__restrict__ V, const double *__restrict__ W,
-                                              RitzCoef cf, const double *__restrict__ diag, double *__restrict__ T,
+                                              RitzCoef cf, const double *__restrict__ dcoef,
+                                              const double *__restrict__ diag, double *__restrict__ T,
                                               double *pres, double *pnorm, double *pdots) {
+    // coefficients from the kernel arguments, or (dcoef != null) from the device
+    // Rayleigh-Ritz: y at dcoef[0 .. M), theta at dcoef[16]
     double y[M];
 #pragma unroll
-    for (int i = 0; i < M; ++i) y[i] = cf.y[i];
-    const double th = cf.theta;
+    for (int i = 0; i < M; ++i) y[i] = dcoef ? dcoef[i] : cf.y[i];
+    const double th = dcoef ? dcoef[16] : cf.theta;
     double acc[M + 2];
 #pragma unroll
     for (int i = 0; i < M + 2; ++i) acc[i] = 0.0;
@@ -534,6 +537,129 @@ __global__ void k_row2col(i64 nd, int k, const S *src, D *dst, i64 ld) {
 }
 
 // ---------------------------------------------------------------------------
+// Device Rayleigh-Ritz of the one-root iteration.
+// ---------------------------------------------------------------------------
+constexpr int RR_MAX = 32;
+
+// Eigenpairs of the m x m Rayleigh matrix G (row stride ld, m <= RR_MAX) by cyclic
+// Jacobi with the round-robin pairing (m/2 disjoint rotations per step, m - 1 steps
+// per sweep) in LDS, one 64-thread block. Writes the lowest eigenvector to
+// coef[0 .. m) and its eigenvalue to coef[16] (and to *host_theta, host-visible),
+// the kk lowest eigenvectors column-wise to yk (column j at yk + j m) and their
+// eigenvalues to wk. Sweeps stop when the off-diagonal mass is below 1e-30 of the total.
+__global__ void __launch_bounds__(64) k_rr(const double *G, int ld, int m, int kk, double *coef, double *yk,
+                                           double *wk, double *host_theta) {
+    __shared__ double a[RR_MAX][RR_MAX + 1], v[RR_MAX][RR_MAX + 1];
+    __shared__ double rc[RR_MAX / 2], rs[RR_MAX / 2];
+    __shared__ int rp[RR_MAX / 2], rq[RR_MAX / 2], order[RR_MAX];
+    const int t = threadIdx.x;
+    const int M = (m + 1) & ~1;  // even; for odd m the index m is an inert padding row
+    for (int idx = t; idx < M * M; idx += 64) {
+        const int i = idx / M, j = idx % M;
+        a[i][j] = (i < m && j < m) ? G[i * ld + j] : 0.0;
+        v[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 40 && M > 1; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int idx = t; idx < m * m; idx += 64) {
+            const int i = idx / m, j = idx % m;
+            const double x = a[i][j] * a[i][j];
+            if (fabs(a[i][j]) < 1e100) tot += x;  // k_gupd's placeholder diagonals left out
+            if (i != j) off += x;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            off += __shfl_xor(off, o);
+            tot += __shfl_xor(tot, o);
+        }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int step = 0; step < M - 1; ++step) {
+            if (t < M / 2) {
+                // positions t and M-1-t of the round-robin circle (position 0 fixed)
+                auto at = [&](int pos) { return pos == 0 ? 0 : 1 + (pos - 1 + step) % (M - 1); };
+                const int i1 = at(t), i2 = at(M - 1 - t);
+                const int pp = min(i1, i2), qq = max(i1, i2);
+                double c = 1.0, sn = 0.0;
+                const double apq = a[pp][qq];
+                if (qq < m && apq != 0.0) {
+                    const double tau = (a[qq][qq] - a[pp][pp]) / (2.0 * apq);
+                    const double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    c = 1.0 / sqrt(1.0 + tt * tt);
+                    sn = tt * c;
+                }
+                rp[t] = pp;
+                rq[t] = qq;
+                rc[t] = c;
+                rs[t] = sn;
+            }
+            __syncthreads();
+            for (int idx = t; idx < M * (M / 2); idx += 64) {  // columns p, q of every row
+                const int k = idx / M, r = idx % M, pp = rp[k], qq = rq[k];
+                const double c = rc[k], sn = rs[k];
+                const double arp = a[r][pp], arq = a[r][qq];
+                a[r][pp] = c * arp - sn * arq;
+                a[r][qq] = sn * arp + c * arq;
+            }
+            __syncthreads();
+            for (int idx = t; idx < M * (M / 2); idx += 64) {  // rows p, q; eigenvector columns
+                const int k = idx / M, r = idx % M, pp = rp[k], qq = rq[k];
+                const double c = rc[k], sn = rs[k];
+                const double apr = a[pp][r], aqr = a[qq][r];
+                a[pp][r] = c * apr - sn * aqr;
+                a[qq][r] = sn * apr + c * aqr;
+                const double vrp = v[r][pp], vrq = v[r][qq];
+                v[r][pp] = c * vrp - sn * vrq;
+                v[r][qq] = sn * vrp + c * vrq;
+            }
+            __syncthreads();
+        }
+    }
+    if (t == 0) {
+        for (int i = 0; i < m; ++i) order[i] = i;
+        for (int i = 1; i < m; ++i) {  // stable insertion sort, ascending
+            const int x = order[i];
+            int j = i - 1;
+            while (j >= 0 && a[order[j]][order[j]] > a[x][x]) {
+                order[j + 1] = order[j];
+                --j;
+            }
+            order[j + 1] = x;
+        }
+        coef[16] = a[order[0]][order[0]];
+        *host_theta = coef[16];
+    }
+    __syncthreads();
+    for (int i = t; i < m; i += 64) coef[i] = v[i][order[0]];
+    for (int idx = t; idx < m * kk; idx += 64) {
+        const int j = idx / m, i = idx % m;
+        yk[j * m + i] = v[i][order[j]];
+    }
+    if (t < kk) wk[t] = a[order[t]][order[t]];
+}
+
+// Appends Rayleigh column m0 (col[i] = V[:, i] . W[:, m0], i <= m0) to the device
+// matrix G (row stride ld), symmetrising the diagonal; a column that the
+// normalisation zeroed as dependent (norm pair nn: after <= 1e-10 before) gets a
+// huge diagonal and zero couplings, so that it never becomes the lowest Ritz pair.
+__global__ void k_gupd(double *G, int ld, int m0, const double *col, const double *nn) {
+    const double n0 = sqrt(fmax(nn[0], 0.0)), n1 = sqrt(fmax(nn[1], 0.0));
+    const bool dep = !(n1 > 1e-10 * fmax(n0, 1e-300) && n1 > 1e-300);
+    for (int i = threadIdx.x; i <= m0; i += blockDim.x) {
+        const double x = dep ? (i == m0 ? 1e300 : 0.0) : col[i];
+        G[i * ld + m0] = x;
+        G[m0 * ld + i] = x;
+    }
+}
+
+// G = diag(w[0 .. kk)) (thick restart), the rest of the mmax x mmax block zero.
+__global__ void k_gdiag(double *G, int ld, int mmax, int kk, const double *w) {
+    for (int idx = threadIdx.x; idx < mmax * mmax; idx += blockDim.x) {
+        const int i = idx / mmax, j = idx % mmax;
+        G[i * ld + j] = (i == j && i < kk) ? w[i] : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host: cyclic Jacobi eigensolver for the small symmetric Rayleigh matrix.
 // ---------------------------------------------------------------------------
 static void jacobi_eig(int m, std::vector<double> a, std::vector<double> &w, std::vector<double> &v) {
@@ -615,6 +741,8 @@ struct Davidson {
     // squared residual norms, the new G block and the norm pairs of the added
     // columns come back in one copy
     size_t oY, oTH, oYK, oRES, oC1, oC2, oNRM, oPAIR, oG, osz;
+    size_t oCF, oYKD, oWK, oGD, oGCOL, oPAIRD, oRING;
+    static constexpr int kRing = 4;
     size_t pRES, pN0, pN1, pN2, pC1, pC2;  // in pb: nblk entries per block-partial quantity
     static constexpr int kCgsMax = 128;    // basis size up to which Gram-Schmidt runs on partials
 
@@ -641,7 +769,17 @@ struct Davidson {
         oNRM = oC2 + M;
         oPAIR = oNRM + 3 * 16;
         oG = oPAIR + 2 * 16;
-        osz = oG + M * 16;
+        // device Rayleigh-Ritz: coefficients (y[0..16), theta at 16), the kk = 2
+        // lowest eigenvectors and values, G, the new Rayleigh column, its norm pair;
+        // host-visible ring of (squared residual norm, theta) per pipelined iteration
+        oCF = oG + M * 16;
+        oYKD = oCF + 17;
+        oWK = oYKD + 2 * M;
+        oGD = oWK + 2;
+        oGCOL = oGD + M * M;
+        oPAIRD = oGCOL + M;
+        oRING = oPAIRD + 2;
+        osz = oRING + 2 * kRing;
         dv.alloc(osz);
         hb.alloc(osz);
         PE_HIP(hipHostGetDevicePointer((void **)&hd, hb.p, 0));
@@ -798,9 +936,11 @@ struct Davidson {
     // more than half the squared norm) and normalised, all on the device; the norm
     // pair (before, after) goes to the slot's pair for the host's dependency check
     // and a dependent vector is zeroed.
-    void orthonormalize_device(int slot, bool have_c1 = false) {
+    void orthonormalize_device(int slot, bool have_c1 = false, double *pair_dst = nullptr,
+                               double *res_dst = nullptr) {
         double *col = V.p + (i64)m * nd;
-        double *pair = hd + oPAIR + 2 * slot;
+        double *pair = pair_dst ? pair_dst : hd + oPAIR + 2 * slot;
+        double *res_out = res_dst ? res_dst : hd + oRES;
         const double *p0 = pb.p + pN0;
         double *p1 = pb.p + pN1, *p2 = pb.p + pN2;
         if (m == 0) {
@@ -810,7 +950,7 @@ struct Davidson {
             if (!have_c1) dots(V.p, m, col, pb.p + pC1, nullptr, nullptr, hnb());
             // after the fused Ritz pass, the first update also finishes its residual norm
             if (have_c1)
-                update(m, pb.p + pC1, col, p1, nullptr, nullptr, pb.p + pRES, hd + oRES);
+                update(m, pb.p + pC1, col, p1, nullptr, nullptr, pb.p + pRES, res_out);
             else
                 update(m, pb.p + pC1, col, p1, nullptr, nullptr);
             dots(V.p, m, col, pb.p + pC2, p0, p1, hnb());
@@ -840,6 +980,13 @@ struct Davidson {
         }
         PE_LAUNCH_CHECK();
         ++m;
+    }
+
+    // orthonormalize_device for the device-driven loop: the norm pair to pair_dst
+    // (device); after the fused Ritz pass (res_dst != null) the residual norm of
+    // that pass is finished into res_dst.
+    void orthonormalize_device_to(double *res_dst, double *pair_dst) {
+        orthonormalize_device(0, res_dst != nullptr, pair_dst, res_dst);
     }
 
     // Block Gram-Schmidt of the cnt corrections in V[:, m0 .. m0 + cnt): twice
@@ -1008,11 +1155,18 @@ struct Davidson {
         for (int i = 0; i < m; ++i) cf.y[i] = hb.p[oY + i] = y[(size_t)i * m];
         cf.theta = hb.p[oTH] = theta[0];
         y_dev = false;  // in hb only (finish_x uploads them when it needs them)
+        launch_ritz1(cf, nullptr);
+    }
+
+    // The same pass with the coefficients the device Rayleigh-Ritz wrote (k_rr).
+    void ritz_spec_dev() { launch_ritz1(RitzCoef{}, dv.p + oCF); }
+
+    void launch_ritz1(const RitzCoef &cf, const double *dcoef) {
         double *t = V.p + (i64)m * nd;
         switch (m) {
 #define PE_R1(M)                                                                                               \
     case M:                                                                                                    \
-        k_ritz1<M><<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, cf, dg, t, pb.p + pRES, pb.p + pN0,           \
+        k_ritz1<M><<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, cf, dcoef, dg, t, pb.p + pRES, pb.p + pN0,    \
                                                  pb.p + pC1);                                                  \
         break;
             PE_R1(1) PE_R1(2) PE_R1(3) PE_R1(4) PE_R1(5) PE_R1(6) PE_R1(7) PE_R1(8)
@@ -1090,13 +1244,171 @@ struct Davidson {
         m = kk;
     }
 
+    // Thick restart on the kk lowest Ritz vectors with the coefficients and values
+    // of the device Rayleigh-Ritz (k_rr); the device G becomes diag(w).
+    void restart_device(int kk) {
+        if (!RV.p) {
+            RV.alloc((size_t)(nd * 2 * k));
+            RW.alloc((size_t)(nd * 2 * k));
+        }
+        k_gemm_set<<<g1(), 256, 0, s>>>(nd, V.p, nd, m, dv.p + oYKD, kk, RV.p, nd);
+        PE_LAUNCH_CHECK();
+        k_gemm_set<<<g1(), 256, 0, s>>>(nd, W.p, nd, m, dv.p + oYKD, kk, RW.p, nd);
+        PE_LAUNCH_CHECK();
+        put_basis(RV.p, kk);
+        PE_HIP(hipMemcpyAsync(W.p, RW.p, sizeof(double) * nd * kk, hipMemcpyDeviceToDevice, s));
+        k_gdiag<<<1, 256, 0, s>>>(dv.p + oGD, mmax, mmax, kk, dv.p + oWK);
+        PE_LAUNCH_CHECK();
+        m = kk;
+    }
+
+    // One-root Davidson with the Rayleigh-Ritz eigenproblem on the device, so that
+    // iterations are queued without waiting for the host: iteration i is enqueued,
+    // then the host reads iteration i-1's residual norm and Ritz value from a
+    // host-visible ring slot (written by the kernels) behind that iteration's event
+    // and decides convergence, while the device runs iteration i. The run ends with
+    // the Ritz pair of the last queued iteration (at least as converged as the one
+    // that met the tolerance). Same iteration as run(): fused Ritz pass and
+    // correction while the basis has room, thick restart on the 2 lowest Ritz
+    // vectors otherwise; dependent corrections are excluded on the device (k_gupd).
+    bool run_device(double tol, i64 maxiter, bool fp32, double switch_tol, std::vector<double> &theta, i64 &iters) {
+        const double eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
+        // the device G from the host's
+        for (int i = 0; i < mmax * mmax; ++i) hb.p[oGD + (size_t)i] = 0.0;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) hb.p[oGD + (size_t)(i * mmax + j)] = G[(size_t)i * mmax + j];
+        PE_HIP(hipMemcpyAsync(dv.p + oGD, hb.p + oGD, sizeof(double) * mmax * mmax, hipMemcpyHostToDevice, s));
+        PE_HIP(hipStreamSynchronize(s));  // hb's staging region is reused below
+        struct Slot {
+            hipEvent_t ev = nullptr;
+            int mx = 0;
+            bool restarted = false;
+        };
+        Slot ring[kRing];
+        for (auto &r : ring) PE_HIP(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
+        auto destroy = [&] {
+            for (auto &r : ring)
+                if (r.ev) (void)hipEventDestroy(r.ev);
+        };
+        double best_res = 1e300;
+        int stall = 0;
+        i64 queued = 0, checked = 0;
+        bool stop = false, ok = false;
+        try {
+            while (!stop) {
+                if (queued - checked < 2 && queued < maxiter + 1) {
+                    // ---- queue iteration `queued` ----
+                    Slot &sl = ring[queued % kRing];
+                    double *hslot = hd + oRING + 2 * (queued % kRing);
+                    const bool room = m + 1 <= mmax;
+                    const int kk = room ? 1 : std::max(k, std::min({m, 2 * k, mmax - 1}));
+                    k_rr<<<1, 64, 0, s>>>(dv.p + oGD, mmax, m, kk, dv.p + oCF, dv.p + oYKD, dv.p + oWK, hslot + 1);
+                    PE_LAUNCH_CHECK();
+                    sl.mx = m;
+                    sl.restarted = !room;
+                    const int m0 = room ? m : kk;
+                    if (room) {
+                        ritz_spec_dev();
+                        orthonormalize_device_to(hslot, dv.p + oPAIRD);
+                    } else {
+                        // residual and correction of the Ritz pair of the full basis, then restart
+                        k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, m, dv.p + oCF, dv.p + oCF + 16, 1, dg, 0,
+                                                            X.p, WX.p, T.p, pb.p + pRES);
+                        PE_LAUNCH_CHECK();
+                        k_reduce_partials<<<1, 256, 0, s>>>(pb.p + pRES, nblk, 1, hslot);
+                        PE_LAUNCH_CHECK();
+                        restart_device(kk);
+                        put_correction(0);
+                        orthonormalize_device_to(nullptr, dv.p + oPAIRD);
+                    }
+                    apply(m0, 1, fp32);
+                    dots_to(V.p, m0 + 1, W.p + (i64)m0 * nd, dv.p + oGCOL, sector != 0 && exact);
+                    k_gupd<<<1, 64, 0, s>>>(dv.p + oGD, mmax, m0, dv.p + oGCOL, dv.p + oPAIRD);
+                    PE_LAUNCH_CHECK();
+                    PE_HIP(hipEventRecord(sl.ev, s));
+                    ++queued;
+                    ++st.iterations;
+                    continue;
+                }
+                // ---- check iteration `checked` ----
+                Slot &sl = ring[checked % kRing];
+                PE_HIP(hipEventSynchronize(sl.ev));
+                const double *hslot = hb.p + oRING + 2 * (checked % kRing);
+                const double res = std::sqrt(std::max(hslot[0], 0.0)), th = hslot[1];
+                ++checked;
+                ++iters;
+                const double lim = tol * std::max(std::fabs(th), eps23);
+                if (res <= lim) {
+                    ok = stop = true;
+                } else if (stop_iters >= 0 && iters >= stop_iters && th - res > stop_above) {
+                    stopped_early = ok = stop = true;
+                } else if (fp32 && res <= std::max(lim, switch_tol * std::max(std::fabs(th), 1.0))) {
+                    stop = true;
+                } else if (checked >= maxiter) {
+                    stop = true;
+                } else {
+                    if (res < 0.9 * best_res) {
+                        best_res = res;
+                        stall = 0;
+                    } else {
+                        ++stall;
+                    }
+                    if (fp32 && stall >= 8) stop = true;
+                    if (!fp32 && stall >= 25) {
+                        if (res <= 1e-7 * std::max(std::fabs(th), 1.0)) ok = stop = true;
+                        else if (stall >= 300) stop = true;
+                    }
+                }
+            }
+            // the last queued iteration's Ritz pair
+            PE_HIP(hipStreamSynchronize(s));
+            const i64 last = queued - 1;
+            const Slot &sl = ring[last % kRing];
+            const double *hslot = hb.p + oRING + 2 * (last % kRing);
+            theta.assign(1, hslot[1]);
+            st.residual[0] = std::sqrt(std::max(hslot[0], 0.0));
+            if (sl.restarted) {
+                PE_HIP(hipMemcpyAsync(X.p, V.p, sizeof(double) * nd, hipMemcpyDeviceToDevice, s));
+                PE_HIP(hipMemcpyAsync(WX.p, W.p, sizeof(double) * nd, hipMemcpyDeviceToDevice, s));
+            } else {
+                k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, sl.mx, dv.p + oCF, dv.p + oCF + 16, 1, dg, 1,
+                                                    X.p, WX.p, T.p, pb.p + pRES);
+                PE_LAUNCH_CHECK();
+            }
+            PE_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            destroy();
+            throw;
+        }
+        destroy();
+        // the host copy of the Rayleigh matrix no longer matches the basis
+        std::fill(G.begin(), G.end(), 0.0);
+        return ok;
+    }
+
     // Runs Davidson until every root meets tol (or maxiter iterations). With
     // fp32 set it stops early once residuals reach the FP32 floor (switch_tol).
     // Returns true when converged to tol; X, WX then hold the Ritz pairs.
     // One root: the correction of an iteration is orthogonalised, multiplied and
     // projected before its residual is known, so that an iteration needs a single
     // host round trip (the last iteration's correction is discarded).
+    // Early stop of a one-root run in the second parity sector (stop_iters >= 0):
+    // the run ends, reporting convergence, once it has taken at least stop_iters
+    // iterations (as many as the first sector needed) and its lowest Ritz value
+    // minus its residual norm lies above stop_above (the first sector's converged
+    // root), so that the sector cannot hold the lowest root at the resolution of
+    // this search. stopped_early records that this happened.
+    double stop_above = 0.0;
+    i64 stop_iters = -1;
+    bool stopped_early = false;
+    bool device_rr = true;  // one-root runs take run_device (false: the host loop)
+
     bool run(double tol, i64 maxiter, bool fp32, double switch_tol, std::vector<double> &theta, i64 &iters) {
+        // one root on an operator whose sigma keeps the sector exactly: the
+        // Rayleigh-Ritz runs on the device and iterations are queued ahead
+        if (k == 1 && exact && mmax <= 17 && m >= 1 && nd > mmax + 1 && device_rr)
+            return run_device(tol, maxiter, fp32, switch_tol, theta, iters);
         const double eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
         std::vector<double> best_res(k, 1e300);
         int stall = 0;
@@ -1148,6 +1460,10 @@ struct Davidson {
                 return ok;
             };
             if (all) return leave(true);
+            if (k == 1 && stop_iters >= 0 && iters >= stop_iters && theta[0] - res[0] > stop_above) {
+                stopped_early = true;
+                return leave(true);
+            }
             if (fp32 && floor) return leave(false);
             if (m0 >= nd) {
                 // the basis spans the whole space: the Ritz pairs are exact to rounding
@@ -1228,13 +1544,16 @@ struct Davidson {
 // != 0). Returns converged flag; theta (k) ascending and the Ritz vectors (k x nd,
 // host) of the run.
 static bool solve_run(pyci_op *op, int k, const double *c0, i64 mm, i64 maxit, double tol, int precision, int sector,
-                      std::vector<double> &theta, std::vector<double> &hx, pyci_solve_stats &st) {
+                      std::vector<double> &theta, std::vector<double> &hx, pyci_solve_stats &st,
+                      double stop_above = 0.0, i64 stop_iters = -1, bool *stopped_early = nullptr) {
     const i64 nd = op->nrow;
     hipStream_t s = op->ctx->stream;
     const i64 nb = op->swap_nb;
     auto partner = [&](i64 J) { return (J % nb) * nb + J / nb; };
     Davidson D(op, k, (int)mm);
     D.set_sector(sector, nb);
+    D.stop_above = stop_above;
+    D.stop_iters = stop_iters;
     std::vector<double> hd((size_t)nd);
     PE_HIP(hipMemcpyAsync(hd.data(), op->diag.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
@@ -1314,25 +1633,31 @@ static bool solve_run(pyci_op *op, int k, const double *c0, i64 mm, i64 maxit, d
         // the FP32 phase runs to 2e-6 |theta| (an FP32 sigma is good to ~1e-7 relative,
         // so this is well above its floor; a stall of 8 iterations also hands over)
         D.run(tol, maxit, true, 2e-6, theta, iters);
-        // FP64 residual-correction phase: restart on the FP32-phase Ritz vectors
-        // with exact FP64 images.
-        PE_HIP(hipMemcpyAsync(D.V.p, D.X.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
-        D.m = 0;
-        for (int j = 0; j < k; ++j) {
-            if (j != D.m)
-                PE_HIP(hipMemcpyAsync(D.V.p + (i64)D.m * nd, D.V.p + (i64)j * nd, sizeof(double) * nd,
-                                      hipMemcpyDeviceToDevice, s));
-            if (D.orthonormalize_new(1.0)) ++D.m;
+        if (D.stopped_early) {
+            // the sector search ended early: no FP64 phase is needed
+            ok = true;
+        } else {
+            // FP64 residual-correction phase: restart on the FP32-phase Ritz vectors
+            // with exact FP64 images.
+            PE_HIP(hipMemcpyAsync(D.V.p, D.X.p, sizeof(double) * nd * k, hipMemcpyDeviceToDevice, s));
+            D.m = 0;
+            for (int j = 0; j < k; ++j) {
+                if (j != D.m)
+                    PE_HIP(hipMemcpyAsync(D.V.p + (i64)D.m * nd, D.V.p + (i64)j * nd, sizeof(double) * nd,
+                                          hipMemcpyDeviceToDevice, s));
+                if (D.orthonormalize_new(1.0)) ++D.m;
+            }
+            PE_REQUIRE(D.m == k, PYCI_ENOCONV, "did not converge");
+            std::fill(D.G.begin(), D.G.end(), 0.0);
+            D.apply(0, D.m, false);
+            D.update_G(0, D.m);
+            ok = D.run(tol, maxit, false, 0.0, theta, iters);
         }
-        PE_REQUIRE(D.m == k, PYCI_ENOCONV, "did not converge");
-        std::fill(D.G.begin(), D.G.end(), 0.0);
-        D.apply(0, D.m, false);
-        D.update_G(0, D.m);
-        ok = D.run(tol, maxit, false, 0.0, theta, iters);
     } else {
         ok = D.run(tol, maxit, false, 0.0, theta, iters);
     }
     st = D.st;
+    if (stopped_early) *stopped_early = D.stopped_early;
     hx.resize((size_t)(nd * k));
     PE_HIP(hipMemcpyAsync(hx.data(), D.X.p, sizeof(double) * nd * k, hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
@@ -1369,7 +1694,15 @@ void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, 
     bool ok_a, ok_b = true;
     if (sectors) {
         ok_a = solve_run(op, k, c0, mm, maxit, tol, precision, +1, theta_a, hx_a, st_a);
-        ok_b = solve_run(op, k, c0, mm, maxit, tol, precision, -1, theta_b, hx_b, st_b);
+        // one root: the second sector is searched for as many iterations as the first
+        // needed and then only as long as its lowest Ritz value, less its residual
+        // norm, is not above the first sector's root
+        bool early = false;
+        if (k == 1 && ok_a)
+            ok_b = solve_run(op, k, c0, mm, maxit, tol, precision, -1, theta_b, hx_b, st_b, theta_a[0],
+                             st_a.iterations, &early);
+        else
+            ok_b = solve_run(op, k, c0, mm, maxit, tol, precision, -1, theta_b, hx_b, st_b);
         for (int j = 0; j < k; ++j) {
             roots.push_back({theta_a[(size_t)j], {0, j}});
             roots.push_back({theta_b[(size_t)j], {1, j}});

@@ -25,6 +25,8 @@
 #include <type_traits>
 #include <vector>
 
+#include <mutex>
+
 #include "internal.hpp"
 
 namespace pe {

@@ -329,6 +329,8 @@ void fill_all_dets_device(pyci_ctx *ctx, int kind, i64 nb, i64 nup, i64 ndn, u64
 // HCI / RDMs
 i64 add_hci_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, double eps);
 void rdms_device(pyci_wfn *wfn, const double *coeffs_host, double *rdm1, double *rdm2);
+void trdms_device(pyci_wfn *w1, pyci_wfn *w2, const double *c1_host, const double *c2_host, double *rdm1,
+                  double *rdm2);
 // ENPT2 correction sum_a (sum_i H_ai c_i)^2 / (e - H_aa), e = energy - ecore (hci.hip)
 double enpt2_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, double e, double eps, i64 *nterms);
 
@@ -376,6 +378,17 @@ void batch_solve(pyci_ctx *ctx, i64 nprob, const i64 *nbasis, const i64 *nup, co
                  const double *const *one_mo, const double *const *two_mo, i64 nroot, double tol, int precision,
                  i64 maxiter, double *evals, double *latency_ms, i64 *iters, i64 *nsigma, int *converged,
                  pyci_batch_stats *stats);
+
+// FanCI geminal objectives (fanci.hip)
+pyci_fanci *fanci_create(pyci_op *op, pyci_wfn *wfn, int kind, i64 ndc, const i64 *idc, const double *vdc, i64 npc,
+                         const i64 *ipc, const double *vpc);
+void fanci_info(const pyci_fanci *F, i64 *nparam, i64 *nproj, i64 *nconn, i64 *nequation);
+int fanci_device(const pyci_fanci *F);
+void fanci_destroy(pyci_fanci *F);
+void fanci_overlap(pyci_fanci *F, const double *x, double *y);
+void fanci_d_overlap(pyci_fanci *F, const double *x, double *y);
+void fanci_objective(pyci_fanci *F, const double *x, double *y);
+void fanci_jacobian(pyci_fanci *F, const double *x, double *y);
 
 // Learned selector (mlp.hip)
 void mlp_init(pyci_mlp *m, int nfeat, u64 seed);

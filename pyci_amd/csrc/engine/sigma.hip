@@ -26,10 +26,30 @@ __device__ inline i64 xcd_row_block(i64 b, i64 nblocks) {
     return r;
 }
 
-template <int NV, typename VT, typename XT>
+// Row-major X with NV values per column; the NV values of one column are read as
+// 16-byte pieces (NV * sizeof(XT) / 16 of them), and every lane issues the
+// column/value loads of U entries (non-temporal: the CSR is streamed once per
+// sigma) before their U * NV gathers, so each wavefront keeps U * 64 entries in flight.
+template <typename XT>
+struct Vec16;
+template <>
+struct Vec16<double> {
+    typedef double2 type;
+    static constexpr int n = 2;
+};
+template <>
+struct Vec16<float> {
+    typedef float4 type;
+    static constexpr int n = 4;
+};
+
+template <int NV, int U, typename VT, typename XT>
 __global__ void __launch_bounds__(256) k_spmm(i64 nrow, i64 nblocks, const i64 *__restrict__ rowptr,
                                               const int32_t *__restrict__ cols, const VT *__restrict__ vals,
                                               const XT *__restrict__ X, XT *__restrict__ Y) {
+    typedef typename Vec16<XT>::type V16;
+    constexpr int PER = Vec16<XT>::n, NQ = NV / PER;  // XT values per piece, pieces per column
+    static_assert(NV % PER == 0, "NV must fill whole 16-byte pieces");
     const int lane = threadIdx.x & 63;
     const i64 blk = xcd_row_block(blockIdx.x, nblocks);
     const i64 row = blk * SPMV_WAVES + (threadIdx.x >> 6);
@@ -38,19 +58,42 @@ __global__ void __launch_bounds__(256) k_spmm(i64 nrow, i64 nblocks, const i64 *
     XT acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] = (XT)0;
+    const V16 *__restrict__ X16 = reinterpret_cast<const V16 *>(X);
     i64 p = beg + lane;
-    // two entries in flight per lane per iteration
-    for (; p + 64 < end; p += 128) {
-        const int c0 = cols[p], c1 = cols[p + 64];
-        const XT v0 = (XT)vals[p], v1 = (XT)vals[p + 64];
+    for (; p + (U - 1) * 64 < end; p += U * 64) {
+        int c[U];
+        VT v[U];
 #pragma unroll
-        for (int j = 0; j < NV; ++j) acc[j] += v0 * X[(i64)c0 * NV + j] + v1 * X[(i64)c1 * NV + j];
+        for (int u = 0; u < U; ++u) {
+            c[u] = __builtin_nontemporal_load(&cols[p + u * 64]);
+            v[u] = __builtin_nontemporal_load(&vals[p + u * 64]);
+        }
+        V16 x[U][NQ];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) x[u][q] = X16[(i64)c[u] * NQ + q];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const XT w = (XT)v[u];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const XT *e = reinterpret_cast<const XT *>(&x[u][q]);
+#pragma unroll
+                for (int t = 0; t < PER; ++t) acc[q * PER + t] += w * e[t];
+            }
+        }
     }
-    if (p < end) {
-        const int c0 = cols[p];
-        const XT v0 = (XT)vals[p];
+    for (; p < end; p += 64) {
+        const int c = __builtin_nontemporal_load(&cols[p]);
+        const XT w = (XT)__builtin_nontemporal_load(&vals[p]);
 #pragma unroll
-        for (int j = 0; j < NV; ++j) acc[j] += v0 * X[(i64)c0 * NV + j];
+        for (int q = 0; q < NQ; ++q) {
+            const V16 xv = X16[(i64)c * NQ + q];
+            const XT *e = reinterpret_cast<const XT *>(&xv);
+#pragma unroll
+            for (int t = 0; t < PER; ++t) acc[q * PER + t] += w * e[t];
+        }
     }
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
@@ -148,9 +191,15 @@ static void spmm(const pyci_op *op, const VT *vals, int nvec, const XT *X, XT *Y
     const int32_t *c = op->cols.p;
     switch (nvec) {
         case 1: k_spmv<4, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
-        case 2: k_spmm<2, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
-        case 4: k_spmm<4, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
-        case 8: k_spmm<8, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
+        case 2:
+            // two FP64 values are one 16-byte piece; two FP32 values are not
+            if constexpr (sizeof(XT) == 8)
+                k_spmm<2, 4, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y);
+            else
+                k_spmm_any<VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, nvec, rp, c, vals, X, Y);
+            break;
+        case 4: k_spmm<4, 4, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
+        case 8: k_spmm<8, 4, VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, rp, c, vals, X, Y); break;
         default: k_spmm_any<VT, XT><<<(unsigned)grid, 256, 0, s>>>(nrow, nblocks, nvec, rp, c, vals, X, Y); break;
     }
     PE_LAUNCH_CHECK();

@@ -241,3 +241,68 @@ def test_enpt2_brute_force(oracle, kind, eps):
     e, nt = oracle.enpt2(kind, nb, nu, nd, dets, one, two, c, energy, eps)
     assert nt == nt_want
     npt.assert_allclose(e - energy, want, rtol=1e-11)
+
+
+@pytest.mark.parametrize("kind,nb,nu,nd", [("fullci", 5, 2, 2), ("doci", 6, 3, 3), ("genci", 8, 3, 0)])
+def test_transition_rdms_reduce_to_rdms(oracle, kind, nb, nu, nd):
+    """<Psi| ... |Psi> through the transition routine equals compute_rdms."""
+    dets = oracle.all_dets(kind, nb, nu, nd)
+    c = np.random.default_rng(21).normal(size=len(dets))
+    c /= np.linalg.norm(c)
+    a1, a2 = oracle.transition_rdms(kind, nb, dets, dets, c, c)
+    r1, r2 = oracle.rdms(kind, nb, dets, c)
+    npt.assert_allclose(a1, r1, rtol=0, atol=1e-13)
+    npt.assert_allclose(a2, r2, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("kind", ["fullci", "doci", "genci"])
+def test_transition_rdms_brute_force(oracle, kind):
+    """Two different sets (overlapping, each missing part of the space) and two
+    coefficient vectors: every element equals <1| a+_p a_q |2> and
+    <1| a+_p a+_q a_s a_r |2> from Jordan-Wigner operator matrices."""
+    rng = np.random.default_rng(22)
+    if kind == "fullci":
+        nb, nu, nd = 4, 2, 1
+        alld = oracle.all_dets("fullci", nb, nu, nd)
+        bits = jw.fullci_basis(alld, nb)
+        ns = 2 * nb
+    elif kind == "doci":
+        nb, nu, nd = 5, 2, 2
+        alld = oracle.all_dets("doci", nb, nu, nd)
+        bits = jw.doci_basis(alld, nb)
+        ns = 2 * nb
+    else:
+        nb, nu, nd = 6, 3, 0
+        alld = oracle.all_dets("genci", nb, nu, nd)
+        bits = alld.astype(np.int64)
+        ns = nb
+    m = len(alld)
+    i1 = np.sort(rng.choice(m, size=m - 3, replace=False))
+    i2 = rng.permutation(np.sort(rng.choice(m, size=m - 4, replace=False)))
+    c1, c2 = rng.normal(size=i1.size), rng.normal(size=i2.size)
+    a1, a2 = oracle.transition_rdms(kind, nb, alld[i1], alld[i2], c1, c2)
+    v1, v2 = np.zeros(m), np.zeros(m)
+    v1[i1], v2[i2] = c1, c2
+    one = lambda p, q: v1 @ jw.operator_matrix([(p, True), (q, False)], bits) @ v2
+    two = lambda p, q, r, s: v1 @ jw.operator_matrix([(p, True), (q, True), (s, False), (r, False)], bits) @ v2
+    if kind == "genci":
+        for p, q in itertools.product(range(nb), repeat=2):
+            npt.assert_allclose(a1[p, q], one(p, q), atol=1e-13)
+        for p, q, r, s in itertools.product(range(nb), repeat=4):
+            npt.assert_allclose(a2[p, q, r, s], two(p, q, r, s), atol=1e-13)
+    elif kind == "fullci":
+        for sp in (0, 1):
+            o = sp * nb
+            for p, q in itertools.product(range(nb), repeat=2):
+                npt.assert_allclose(a1[sp, p, q], one(p + o, q + o), atol=1e-13)
+        for p, q, r, s in itertools.product(range(nb), repeat=4):
+            npt.assert_allclose(a2[0, p, q, r, s], two(p, q, r, s), atol=1e-13)
+            npt.assert_allclose(a2[1, p, q, r, s], two(p + nb, q + nb, r + nb, s + nb), atol=1e-13)
+            npt.assert_allclose(a2[2, p, q, r, s], two(p, q + nb, r, s + nb), atol=1e-13)
+    else:
+        # DOCI: d0[p,q] = <1| a+_pa a+_pb a_qb a_qa |2>; d2[p,q] (p != q) = <1| n_pa n_qa |2> (the
+        # seniority-zero pair-pair occupation, rdm.cpp:634-672)
+        for p, q in itertools.product(range(nb), repeat=2):
+            npt.assert_allclose(a1[p, q], two(p, p + nb, q, q + nb), atol=1e-13)
+            if p != q:
+                npt.assert_allclose(a2[p, q], two(p, q, p, q), atol=1e-13)
