@@ -140,6 +140,16 @@ int pyci_op_matvec(pyci_op *op, int64_t nvec, const double *x, double *y);
 /* Same with device pointers (x, y in HBM, X row-major ncol x nvec, i.e. the nvec
  * values of one determinant are contiguous); runs on the context stream. */
 int pyci_op_matvec_device(pyci_op *op, int64_t nvec, const double *x_dev, double *y_dev);
+/* Allocation-free variant of pyci_op_matvec_device. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). Same
+ * arguments and result; it calls no hipMalloc/hipFree. The matrix-free
+ * operator's E workspace is a fixed per-device array of 256 MiB (held from
+ * pyci_ctx_create to process exit): a call that needs more
+ * (kpad * ceil(nrow * nvec / 64) * 64 * 8 bytes) fails with PYCI_EUNSUPPORTED.
+ * Calls that overlap on one device share that array and race; serialise them
+ * (one context stream does). The CSR product needs no workspace. */
+int pyci_op_matvec_device_noalloc(pyci_op *op, int64_t nvec, const double *x_dev, double *y_dev);
 /* Timed sigma for measurement: nrep launches of the SpMM kernel on device-resident
  * random X (nvec columns); returns the average kernel time from HIP events and the
  * algorithmic bytes of one launch. fp32 != 0 times the FP32-value kernel. */

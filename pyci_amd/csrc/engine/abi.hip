@@ -110,6 +110,7 @@ int pyci_ctx_create(int device, pyci_ctx **out) {
                 for (int k = 0; k <= 64; ++k) b[(size_t)(n * 65 + k)] = binom_host(n, k);
             ctx->binom.alloc(65 * 65);
             PE_HIP(hipMemcpy(ctx->binom.p, b.data(), sizeof(i64) * 65 * 65, hipMemcpyHostToDevice));
+            ctx->noalloc_ws = noalloc_scratch(device);
         } catch (...) {
             pyci_ctx_destroy(ctx);
             throw;
@@ -445,6 +446,19 @@ int pyci_op_matvec_device(pyci_op *op, int64_t nvec, const double *x, double *y)
         check_matvec_shape(op);
         set_device(op->ctx);
         sigma_f64(op, (int)nvec, x, y, op->ctx->stream);
+    });
+}
+
+int pyci_op_matvec_device_noalloc(pyci_op *op, int64_t nvec, const double *x, double *y) {
+    return guarded([&] {
+        PE_REQUIRE(op && x && y, PYCI_EINVAL, "null argument");
+        PE_REQUIRE(nvec >= 1, PYCI_EINVAL, "nvec < 1");
+        check_matvec_shape(op);
+        set_device(op->ctx);
+        if (op->storage == PYCI_STORAGE_DIRECT)
+            direct_sigma_noalloc(op, (int)nvec, x, y, op->ctx->noalloc_ws, op->ctx->stream);
+        else
+            sigma_f64(op, (int)nvec, x, y, op->ctx->stream);  // the CSR product uses no workspace
     });
 }
 

@@ -931,6 +931,47 @@ template void mf_sigma_batched<double>(const MfView &, i64, int, const double *,
 template void mf_sigma_batched<float>(const MfView &, i64, int, const float *, float *, float *, const float *,
                                       const float *, const long long *, int, hipStream_t);
 
+// ---------------------------------------------------------------------------
+// Allocation-free sigma: the E workspace in a fixed device array instead of the
+// context's grown-on-demand buffer, for callers that cannot allocate (code under
+// graph capture, serving loops). The array holds kNoallocElems doubles (256 MiB)
+// on every device the library runs on, for the life of the process; calls that
+// overlap on one device share it and race. It needs no clearing: the scatter
+// pass reads only what the product pass of the same call wrote.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoallocElems = (size_t)32 << 20;
+static __device__ __align__(256) double scratch[kNoallocElems];
+
+// Device address of `scratch` on the current device: looked up once per device
+// (the name is only a host-side handle; each device holds its own copy).
+double *noalloc_scratch(int device) {
+    static std::mutex mu;
+    static std::vector<double *> ptrs;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((size_t)device >= ptrs.size()) ptrs.resize((size_t)device + 1, nullptr);
+    if (!ptrs[(size_t)device]) {
+        double *p = nullptr;
+        PE_HIP(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+        ptrs[(size_t)device] = p;
+    }
+    return ptrs[(size_t)device];
+}
+
+size_t noalloc_scratch_bytes() { return kNoallocElems * sizeof(double); }
+
+void direct_sigma_noalloc(const pyci_op *op, int nvec, const double *X, double *Y, double *ws, hipStream_t s) {
+    const DirectOp *D = op->direct.get();
+    PE_REQUIRE(nvec >= 1 && nvec <= 16, PYCI_EUNSUPPORTED, "the matrix-free operator takes 1..16 vectors");
+    const i64 nd = op->nrow;
+    const i64 need = ((nd * nvec + 63) / 64 * 64) * (i64)D->kpad;
+    PE_REQUIRE((size_t)need <= kNoallocElems, PYCI_EUNSUPPORTED,
+               "the allocation-free product needs " + std::to_string(need * 8) + " bytes of workspace for " +
+                   std::to_string(nvec) + " vector(s); its fixed workspace holds " +
+                   std::to_string(kNoallocElems * 8) + " bytes");
+    launch_passes_nv<double>(view_of(D), MfCall{nd, nvec, nvec, 0, 0, nullptr}, X, ws, Y, D->V64.p, D->k64.p, s,
+                             nullptr, 0);
+}
+
 template void direct_sigma<double>(const pyci_op *, int, const double *, double *, hipStream_t, hipEvent_t *, int);
 template void direct_sigma<float>(const pyci_op *, int, const float *, float *, hipStream_t, hipEvent_t *, int);
 

@@ -130,6 +130,7 @@ struct pyci_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     pe::DevBuf<pe::i64> binom;  // C(n, k) for 0 <= n, k <= 64, row-major [n][k] (65 x 65)
     pe::DevBuf<unsigned char> mf_ws;  // E workspace of the matrix-free sigma (shared by the operators)
+    double *noalloc_ws = nullptr;     // the fixed workspace of the allocation-free product on this device
 };
 
 struct pyci_ham {
@@ -365,6 +366,10 @@ void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, 
                   int sector = 0);
 void direct_bench_parts(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *ms, double *bytes, double *flops);
 void direct_ensure_csr(pyci_op *op);
+// Allocation-free matrix-free product (direct.hip): E in a fixed per-device array
+double *noalloc_scratch(int device);
+size_t noalloc_scratch_bytes();
+void direct_sigma_noalloc(const pyci_op *op, int nvec, const double *X, double *Y, double *ws, hipStream_t s);
 // helpers shared with the batched solver (direct.hip)
 std::vector<u64> colex_strings(int n, int k);
 i64 rank_host(u64 det, const std::vector<i64> &binom);
