@@ -151,3 +151,90 @@ def test_batch64_all_problems_match_oracle(batch64, precision):
     npt.assert_allclose(ev[:, ::-1], g["evals"], rtol=0, atol=1e-8)
     lat = np.asarray(info["latency_ms"])
     assert np.all(lat > 0) and np.all(lat <= info["total_ms"])
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.uint64).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def genci24():
+    """48 spin-orbitals, 12 electrons: genci_wfn(48, 12, 0) with an explicit
+    Hartree-Fock reference and its singles and doubles (42,013 determinants)."""
+    from pyci_amd.selection import hartreefock_genci
+    from pyci_amd.synthetic import spin_orbital
+
+    g = dict(np.load(datafile("genci24.npz")))
+    _, h, gg = config_integrals("genci24")
+    one, two = spin_orbital(h, gg)
+    ham = pyci.hamiltonian(0.0, one, two)
+    hf = hartreefock_genci(24, 6, 6)
+
+    def start():
+        w = pyci.genci_wfn(48, 12, 0)
+        pyci.add_excitations(w, 0, 1, 2, ref=hf)
+        return w
+
+    return g, ham, start
+
+
+@pytest.mark.timeout(300)
+def test_genci24_two_heat_bath_steps_match_oracle(genci24):
+    """Two add_hci -> update -> solve steps (eps 1e-3, then 5e-4) on the GenCI
+    workload: each appended set (count, order, SHA-256) equals the oracle's, and
+    each variational energy is within 1e-8 Eh of the oracle's ARPACK root
+    (tests/golden/make_genci24.py; hci.cpp:184-279, sparseop.cpp:429-502)."""
+    g, ham, start = genci24
+    eps1, eps2, _ = g["eps"]
+    wfn = start()
+    d0 = wfn.to_det_array().reshape(-1)
+    assert d0.size == int(g["counts"][0]) and _sha(d0) == str(g["sha"][0])
+    op = pyci.sparse_op(ham, wfn)
+    es, _ = op.solve(n=1, tol=1e-10)
+    assert abs(es[0] - g["energies"][0]) < 1e-8
+    n0 = len(wfn)
+    added = pyci.add_hci(ham, wfn, g["c0"].astype(np.float64), eps=eps1)
+    assert added == int(g["counts"][1])
+    d1 = wfn.to_det_array().reshape(-1)
+    assert _sha(d1[n0:]) == str(g["sha"][1])
+    op.update(ham, wfn)
+    es, _ = op.solve(n=1, tol=1e-10)
+    assert abs(es[0] - g["energies"][1]) < 1e-8
+    n1 = len(wfn)
+    added = pyci.add_hci(ham, wfn, g["c1"].astype(np.float64), eps=eps2)
+    assert added == int(g["counts"][2])
+    assert _sha(wfn.to_det_array().reshape(-1)[n1:]) == str(g["sha"][2])
+    op.update(ham, wfn)
+    es, _ = op.solve(n=1, tol=1e-10)
+    assert abs(es[0] - g["energies"][2]) < 1e-8
+
+
+@pytest.mark.timeout(300)
+def test_genci24_learned_step_selects_from_oracle_pool(genci24):
+    """The learned selector's step from the same start: its candidate pool (plain
+    add_hci at eps / pool) equals the oracle's, it appends as many determinants as
+    the plain step, and they are the network's top scores over the pool, in
+    pool order; its energy is below the start space's and is printed beside the plain one."""
+    from oracle import mlp as M
+
+    g, ham, start = genci24
+    eps1, _, pool = g["eps"]
+    c0 = g["c0"].astype(np.float64)
+    w_pool = start()
+    n0 = len(w_pool)
+    assert pyci.add_hci(ham, w_pool, c0, eps=eps1 / pool) == int(g["counts"][3])
+    cand = w_pool.to_det_array().reshape(-1)[n0:]
+    assert _sha(cand) == str(g["sha"][3])
+    wfn = start()
+    sel = pyci.mlp_selector(48, seed=0)
+    loss = sel.train(wfn, c0, steps=200, batch=8192, lr=1e-3, cmin=1e-10, seed=0)
+    assert np.isfinite(loss)
+    y = np.asarray(sel.predict(w_pool), dtype=np.float32)[n0:]
+    expect = cand[M.topk_select(y, int(g["counts"][1]))]
+    added = pyci.add_hci(ham, wfn, c0, eps=eps1, selector=sel, pool=pool)
+    assert added == int(g["counts"][1])
+    npt.assert_array_equal(wfn.to_det_array().reshape(-1)[n0:], expect)
+    op = pyci.sparse_op(ham, wfn)
+    es, _ = op.solve(n=1, tol=1e-10)
+    print(f"genci24 step 1 at {len(wfn)} dets: learned {es[0]:.10f}, plain {g['energies'][1]:.10f}")
+    assert es[0] <= g["energies"][0] + 1e-9  # the start space is a subset
