@@ -59,6 +59,8 @@ WORKLOADS = {
     "fci14": dict(name="fci14", nbasis=14, nup=7, ndn=7, integrals="fci14", nroot=8, csr=False),
     # 64 uneven FullCI problems (nbasis 8..12, 3..6 electrons of each spin), 2 roots each
     "batch64": dict(name="batch64", nroot=2, batch=64),
+    # selected CI over 48 spin-orbitals, 12 electrons: plain and learned heat-bath trajectories
+    "genci24": dict(name="genci24", nroot=1, selection=True),
 }
 
 
@@ -168,6 +170,70 @@ def bench_batch(args, wl):
         "rows_per_block": int(res[-1][6].rows_per_block),
         "converged": int(min(r[5].sum() for r in res)),
         "max_abs_dE": max(r[7] for r in res),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_selection(args, wl):
+    """Selected CI on the GenCI workload of 48 spin-orbitals and 12 electrons
+    (24 spatial orbitals, 64 decaying vectors, seed 1003): from HF + singles +
+    doubles (42,013 determinants), add_hci -> sparse_op -> solve at eps = 1e-3,
+    5e-4, 2e-4, 1e-4, 5e-5, 2e-5, 1e-5, stopping once the set holds max_det
+    determinants or the next CSR would exceed the HBM budget. Three runs: plain
+    heat-bath selection, and the learned selector (network retrained on each CI
+    vector) ranking a pool 5x looser than eps, by the network alone (score mode 0)
+    and by network + log10 heat-bath score (1), each appending per step as many
+    determinants as plain selection at that eps would. Prints the (ndet, energy)
+    trajectories and the time of each step."""
+    import pyci_amd as pyci
+    from pyci_amd.selection import hartreefock_genci, run_selection
+    from pyci_amd.synthetic import spin_orbital
+
+    _, h, g = config_integrals("genci24")
+    one, two = spin_orbital(h, g)
+    ham = pyci.hamiltonian(0.0, one, two)
+    hf = hartreefock_genci(24, 6, 6)
+    schedule = [1e-3, 5e-4, 2e-4, 1e-4, 5e-5, 2e-5, 1e-5]
+    max_det = int(os.environ.get("PYCI_BENCH_MAX_DET", 5_000_000))
+    runs = {}
+    for name, kw in (("heat-bath", {}), ("learned-net", dict(score_mode=0)), ("learned-net+hb", dict(score_mode=1))):
+        wfn = pyci.genci_wfn(48, 12, 0)
+        pyci.add_excitations(wfn, 0, 1, 2, ref=hf)
+        sel = pyci.mlp_selector(48, seed=0) if kw else None
+        steps = []
+        t_last = [time.perf_counter()]
+
+        def cb(t):
+            now = time.perf_counter()
+            t["seconds"] = round(now - t_last[0], 3)
+            t_last[0] = now
+            steps.append(t)
+            log(f"{name}: {t['ndet']} dets, E {t['energy']:.10f} (eps {t['eps']:.0e}, +{t['added']}, "
+                f"{t['seconds']:.1f} s)")
+
+        t0 = time.perf_counter()
+        traj = run_selection(ham, wfn, schedule, selector=sel, pool=5.0, train_steps=300, batch=8192,
+                             tol=1e-8, max_det=max_det, callback=cb, **kw)
+        runs[name] = dict(trajectory=[dict(ndet=t["ndet"], energy=round(t["energy"], 10), eps=t["eps"],
+                                           seconds=t.get("seconds")) for t in traj],
+                          total_s=round(time.perf_counter() - t0, 3))
+    hb = runs["heat-bath"]
+    out = {
+        "metric": "selected CI (GenCI 48 spin-orbitals, 12 e-): plain vs learned heat-bath trajectories",
+        "value": hb["total_s"],
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": len(hb["trajectory"]) - 1,
+        "warmup": 0,
+        "ms_per_step": round(1e3 * hb["total_s"] / max(1, len(hb["trajectory"]) - 1), 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded integrals, seed 1003)",
+        "config": {"workload": "genci_wfn(48, 12, 0), HF+SD start, eps 1e-3 .. 1e-5", "max_det": max_det,
+                   "parallelism": "single GPU"},
+        "runs": runs,
     }
     print(json.dumps(out), flush=True)
 
@@ -340,6 +406,9 @@ def main():
     wl = WORKLOADS[args.workload]
     if "batch" in wl:
         bench_batch(args, wl)
+        return
+    if wl.get("selection"):
+        bench_selection(args, wl)
         return
     L = engine()
     ctx = ctypes.c_void_p()

@@ -15,9 +15,12 @@
 // the FP32-value sigma with FP32 accumulation; it hands its Ritz vectors to an
 // FP64 phase (the residual-correction pass) that converges to the FP64 tolerance.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <numeric>
+#include <thread>
 
 #include "internal.hpp"
 
@@ -325,15 +328,12 @@ struct RitzCoef {
 
 template <int M>
 __global__ void __launch_bounds__(DT) k_ritz1(i64 nd, const double *__restrict__ V, const double *__restrict__ W,
-                                              RitzCoef cf, const double *__restrict__ dcoef,
-                                              const double *__restrict__ diag, double *__restrict__ T,
+                                              RitzCoef cf, const double *__restrict__ diag, double *__restrict__ T,
                                               double *pres, double *pnorm, double *pdots) {
-    // coefficients from the kernel arguments, or (dcoef != null) from the device
-    // Rayleigh-Ritz: y at dcoef[0 .. M), theta at dcoef[16]
     double y[M];
 #pragma unroll
-    for (int i = 0; i < M; ++i) y[i] = dcoef ? dcoef[i] : cf.y[i];
-    const double th = dcoef ? dcoef[16] : cf.theta;
+    for (int i = 0; i < M; ++i) y[i] = cf.y[i];
+    const double th = cf.theta;
     double acc[M + 2];
 #pragma unroll
     for (int i = 0; i < M + 2; ++i) acc[i] = 0.0;
@@ -537,129 +537,6 @@ __global__ void k_row2col(i64 nd, int k, const S *src, D *dst, i64 ld) {
 }
 
 // ---------------------------------------------------------------------------
-// Device Rayleigh-Ritz of the one-root iteration.
-// ---------------------------------------------------------------------------
-constexpr int RR_MAX = 32;
-
-// Eigenpairs of the m x m Rayleigh matrix G (row stride ld, m <= RR_MAX) by cyclic
-// Jacobi with the round-robin pairing (m/2 disjoint rotations per step, m - 1 steps
-// per sweep) in LDS, one 64-thread block. Writes the lowest eigenvector to
-// coef[0 .. m) and its eigenvalue to coef[16] (and to *host_theta, host-visible),
-// the kk lowest eigenvectors column-wise to yk (column j at yk + j m) and their
-// eigenvalues to wk. Sweeps stop when the off-diagonal mass is below 1e-30 of the total.
-__global__ void __launch_bounds__(64) k_rr(const double *G, int ld, int m, int kk, double *coef, double *yk,
-                                           double *wk, double *host_theta) {
-    __shared__ double a[RR_MAX][RR_MAX + 1], v[RR_MAX][RR_MAX + 1];
-    __shared__ double rc[RR_MAX / 2], rs[RR_MAX / 2];
-    __shared__ int rp[RR_MAX / 2], rq[RR_MAX / 2], order[RR_MAX];
-    const int t = threadIdx.x;
-    const int M = (m + 1) & ~1;  // even; for odd m the index m is an inert padding row
-    for (int idx = t; idx < M * M; idx += 64) {
-        const int i = idx / M, j = idx % M;
-        a[i][j] = (i < m && j < m) ? G[i * ld + j] : 0.0;
-        v[i][j] = (i == j) ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    for (int sweep = 0; sweep < 40 && M > 1; ++sweep) {
-        double off = 0.0, tot = 0.0;
-        for (int idx = t; idx < m * m; idx += 64) {
-            const int i = idx / m, j = idx % m;
-            const double x = a[i][j] * a[i][j];
-            if (fabs(a[i][j]) < 1e100) tot += x;  // k_gupd's placeholder diagonals left out
-            if (i != j) off += x;
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            off += __shfl_xor(off, o);
-            tot += __shfl_xor(tot, o);
-        }
-        if (off <= 1e-30 * tot || off == 0.0) break;
-        for (int step = 0; step < M - 1; ++step) {
-            if (t < M / 2) {
-                // positions t and M-1-t of the round-robin circle (position 0 fixed)
-                auto at = [&](int pos) { return pos == 0 ? 0 : 1 + (pos - 1 + step) % (M - 1); };
-                const int i1 = at(t), i2 = at(M - 1 - t);
-                const int pp = min(i1, i2), qq = max(i1, i2);
-                double c = 1.0, sn = 0.0;
-                const double apq = a[pp][qq];
-                if (qq < m && apq != 0.0) {
-                    const double tau = (a[qq][qq] - a[pp][pp]) / (2.0 * apq);
-                    const double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    c = 1.0 / sqrt(1.0 + tt * tt);
-                    sn = tt * c;
-                }
-                rp[t] = pp;
-                rq[t] = qq;
-                rc[t] = c;
-                rs[t] = sn;
-            }
-            __syncthreads();
-            for (int idx = t; idx < M * (M / 2); idx += 64) {  // columns p, q of every row
-                const int k = idx / M, r = idx % M, pp = rp[k], qq = rq[k];
-                const double c = rc[k], sn = rs[k];
-                const double arp = a[r][pp], arq = a[r][qq];
-                a[r][pp] = c * arp - sn * arq;
-                a[r][qq] = sn * arp + c * arq;
-            }
-            __syncthreads();
-            for (int idx = t; idx < M * (M / 2); idx += 64) {  // rows p, q; eigenvector columns
-                const int k = idx / M, r = idx % M, pp = rp[k], qq = rq[k];
-                const double c = rc[k], sn = rs[k];
-                const double apr = a[pp][r], aqr = a[qq][r];
-                a[pp][r] = c * apr - sn * aqr;
-                a[qq][r] = sn * apr + c * aqr;
-                const double vrp = v[r][pp], vrq = v[r][qq];
-                v[r][pp] = c * vrp - sn * vrq;
-                v[r][qq] = sn * vrp + c * vrq;
-            }
-            __syncthreads();
-        }
-    }
-    if (t == 0) {
-        for (int i = 0; i < m; ++i) order[i] = i;
-        for (int i = 1; i < m; ++i) {  // stable insertion sort, ascending
-            const int x = order[i];
-            int j = i - 1;
-            while (j >= 0 && a[order[j]][order[j]] > a[x][x]) {
-                order[j + 1] = order[j];
-                --j;
-            }
-            order[j + 1] = x;
-        }
-        coef[16] = a[order[0]][order[0]];
-        *host_theta = coef[16];
-    }
-    __syncthreads();
-    for (int i = t; i < m; i += 64) coef[i] = v[i][order[0]];
-    for (int idx = t; idx < m * kk; idx += 64) {
-        const int j = idx / m, i = idx % m;
-        yk[j * m + i] = v[i][order[j]];
-    }
-    if (t < kk) wk[t] = a[order[t]][order[t]];
-}
-
-// Appends Rayleigh column m0 (col[i] = V[:, i] . W[:, m0], i <= m0) to the device
-// matrix G (row stride ld), symmetrising the diagonal; a column that the
-// normalisation zeroed as dependent (norm pair nn: after <= 1e-10 before) gets a
-// huge diagonal and zero couplings, so that it never becomes the lowest Ritz pair.
-__global__ void k_gupd(double *G, int ld, int m0, const double *col, const double *nn) {
-    const double n0 = sqrt(fmax(nn[0], 0.0)), n1 = sqrt(fmax(nn[1], 0.0));
-    const bool dep = !(n1 > 1e-10 * fmax(n0, 1e-300) && n1 > 1e-300);
-    for (int i = threadIdx.x; i <= m0; i += blockDim.x) {
-        const double x = dep ? (i == m0 ? 1e300 : 0.0) : col[i];
-        G[i * ld + m0] = x;
-        G[m0 * ld + i] = x;
-    }
-}
-
-// G = diag(w[0 .. kk)) (thick restart), the rest of the mmax x mmax block zero.
-__global__ void k_gdiag(double *G, int ld, int mmax, int kk, const double *w) {
-    for (int idx = threadIdx.x; idx < mmax * mmax; idx += blockDim.x) {
-        const int i = idx / mmax, j = idx % mmax;
-        G[i * ld + j] = (i == j && i < kk) ? w[i] : 0.0;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Host: cyclic Jacobi eigensolver for the small symmetric Rayleigh matrix.
 // ---------------------------------------------------------------------------
 static void jacobi_eig(int m, std::vector<double> a, std::vector<double> &w, std::vector<double> &v) {
@@ -713,9 +590,22 @@ static void jacobi_eig(int m, std::vector<double> a, std::vector<double> &w, std
     v.swap(vv);
 }
 
+// What the first parity sector's one-root run tells the second one, which runs
+// concurrently on another stream and host thread: the first run's state (0
+// running, 1 converged, 2 ended unconverged, 3 aborted by an error), its
+// completed iterations while it runs, and once it has converged, its root and
+// iteration count.
+struct SectorLink {
+    std::atomic<int> state{0};
+    std::atomic<i64> done{0};
+    double theta = 0.0;
+    i64 iters = 0;
+};
+
 struct Davidson {
     pyci_op *op;
     hipStream_t s;
+    DevBuf<unsigned char> *ws;  // the matrix-free sigma's E workspace (nullptr: the context's)
     i64 nd;
     int k, mmax;
     DevBuf<double> V, W, X, WX, T, rowbuf, rowout, partial, small;
@@ -741,12 +631,11 @@ struct Davidson {
     // squared residual norms, the new G block and the norm pairs of the added
     // columns come back in one copy
     size_t oY, oTH, oYK, oRES, oC1, oC2, oNRM, oPAIR, oG, osz;
-    size_t oCF, oYKD, oWK, oGD, oGCOL, oPAIRD, oRING;
-    static constexpr int kRing = 4;
     size_t pRES, pN0, pN1, pN2, pC1, pC2;  // in pb: nblk entries per block-partial quantity
     static constexpr int kCgsMax = 128;    // basis size up to which Gram-Schmidt runs on partials
 
-    Davidson(pyci_op *o, int k_, int mmax_) : op(o), s(o->ctx->stream), nd(o->nrow), k(k_), mmax(mmax_) {
+    Davidson(pyci_op *o, int k_, int mmax_, hipStream_t stream = nullptr, DevBuf<unsigned char> *ws_ = nullptr)
+        : op(o), s(stream ? stream : o->ctx->stream), ws(ws_), nd(o->nrow), k(k_), mmax(mmax_) {
         V.alloc((size_t)(nd * mmax));
         W.alloc((size_t)(nd * mmax));
         X.alloc((size_t)(nd * k));
@@ -769,17 +658,7 @@ struct Davidson {
         oNRM = oC2 + M;
         oPAIR = oNRM + 3 * 16;
         oG = oPAIR + 2 * 16;
-        // device Rayleigh-Ritz: coefficients (y[0..16), theta at 16), the kk = 2
-        // lowest eigenvectors and values, G, the new Rayleigh column, its norm pair;
-        // host-visible ring of (squared residual norm, theta) per pipelined iteration
-        oCF = oG + M * 16;
-        oYKD = oCF + 17;
-        oWK = oYKD + 2 * M;
-        oGD = oWK + 2;
-        oGCOL = oGD + M * M;
-        oPAIRD = oGCOL + M;
-        oRING = oPAIRD + 2;
-        osz = oRING + 2 * kRing;
+        osz = oG + M * 16;
         dv.alloc(osz);
         hb.alloc(osz);
         PE_HIP(hipHostGetDevicePointer((void **)&hd, hb.p, 0));
@@ -888,16 +767,16 @@ struct Davidson {
             }
             k_col2row<double, float><<<g1(), 256, 0, s>>>(nd, cnt, src, nd, row32.p);
             PE_LAUNCH_CHECK();
-            sigma_f32(op, cnt, row32.p, out32.p, s, sector);
+            sigma_f32(op, cnt, row32.p, out32.p, s, sector, ws);
             k_row2col<float, double><<<g1(), 256, 0, s>>>(nd, cnt, out32.p, dst, nd);
             PE_LAUNCH_CHECK();
             st.sigma_calls_f32 += cnt;
         } else if (cnt == 1) {
-            sigma_f64(op, 1, src, dst, s, sector);
+            sigma_f64(op, 1, src, dst, s, sector, ws);
         } else {
             k_col2row<double, double><<<g1(), 256, 0, s>>>(nd, cnt, src, nd, rowbuf.p);
             PE_LAUNCH_CHECK();
-            sigma_f64(op, cnt, rowbuf.p, rowout.p, s, sector);
+            sigma_f64(op, cnt, rowbuf.p, rowout.p, s, sector, ws);
             k_row2col<double, double><<<g1(), 256, 0, s>>>(nd, cnt, rowout.p, dst, nd);
             PE_LAUNCH_CHECK();
         }
@@ -936,11 +815,9 @@ struct Davidson {
     // more than half the squared norm) and normalised, all on the device; the norm
     // pair (before, after) goes to the slot's pair for the host's dependency check
     // and a dependent vector is zeroed.
-    void orthonormalize_device(int slot, bool have_c1 = false, double *pair_dst = nullptr,
-                               double *res_dst = nullptr) {
+    void orthonormalize_device(int slot, bool have_c1 = false) {
         double *col = V.p + (i64)m * nd;
-        double *pair = pair_dst ? pair_dst : hd + oPAIR + 2 * slot;
-        double *res_out = res_dst ? res_dst : hd + oRES;
+        double *pair = hd + oPAIR + 2 * slot;
         const double *p0 = pb.p + pN0;
         double *p1 = pb.p + pN1, *p2 = pb.p + pN2;
         if (m == 0) {
@@ -950,7 +827,7 @@ struct Davidson {
             if (!have_c1) dots(V.p, m, col, pb.p + pC1, nullptr, nullptr, hnb());
             // after the fused Ritz pass, the first update also finishes its residual norm
             if (have_c1)
-                update(m, pb.p + pC1, col, p1, nullptr, nullptr, pb.p + pRES, res_out);
+                update(m, pb.p + pC1, col, p1, nullptr, nullptr, pb.p + pRES, hd + oRES);
             else
                 update(m, pb.p + pC1, col, p1, nullptr, nullptr);
             dots(V.p, m, col, pb.p + pC2, p0, p1, hnb());
@@ -980,13 +857,6 @@ struct Davidson {
         }
         PE_LAUNCH_CHECK();
         ++m;
-    }
-
-    // orthonormalize_device for the device-driven loop: the norm pair to pair_dst
-    // (device); after the fused Ritz pass (res_dst != null) the residual norm of
-    // that pass is finished into res_dst.
-    void orthonormalize_device_to(double *res_dst, double *pair_dst) {
-        orthonormalize_device(0, res_dst != nullptr, pair_dst, res_dst);
     }
 
     // Block Gram-Schmidt of the cnt corrections in V[:, m0 .. m0 + cnt): twice
@@ -1155,18 +1025,11 @@ struct Davidson {
         for (int i = 0; i < m; ++i) cf.y[i] = hb.p[oY + i] = y[(size_t)i * m];
         cf.theta = hb.p[oTH] = theta[0];
         y_dev = false;  // in hb only (finish_x uploads them when it needs them)
-        launch_ritz1(cf, nullptr);
-    }
-
-    // The same pass with the coefficients the device Rayleigh-Ritz wrote (k_rr).
-    void ritz_spec_dev() { launch_ritz1(RitzCoef{}, dv.p + oCF); }
-
-    void launch_ritz1(const RitzCoef &cf, const double *dcoef) {
         double *t = V.p + (i64)m * nd;
         switch (m) {
 #define PE_R1(M)                                                                                               \
     case M:                                                                                                    \
-        k_ritz1<M><<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, cf, dcoef, dg, t, pb.p + pRES, pb.p + pN0,    \
+        k_ritz1<M><<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, cf, dg, t, pb.p + pRES, pb.p + pN0,           \
                                                  pb.p + pC1);                                                  \
         break;
             PE_R1(1) PE_R1(2) PE_R1(3) PE_R1(4) PE_R1(5) PE_R1(6) PE_R1(7) PE_R1(8)
@@ -1244,176 +1107,45 @@ struct Davidson {
         m = kk;
     }
 
-    // Thick restart on the kk lowest Ritz vectors with the coefficients and values
-    // of the device Rayleigh-Ritz (k_rr); the device G becomes diag(w).
-    void restart_device(int kk) {
-        if (!RV.p) {
-            RV.alloc((size_t)(nd * 2 * k));
-            RW.alloc((size_t)(nd * 2 * k));
-        }
-        k_gemm_set<<<g1(), 256, 0, s>>>(nd, V.p, nd, m, dv.p + oYKD, kk, RV.p, nd);
-        PE_LAUNCH_CHECK();
-        k_gemm_set<<<g1(), 256, 0, s>>>(nd, W.p, nd, m, dv.p + oYKD, kk, RW.p, nd);
-        PE_LAUNCH_CHECK();
-        put_basis(RV.p, kk);
-        PE_HIP(hipMemcpyAsync(W.p, RW.p, sizeof(double) * nd * kk, hipMemcpyDeviceToDevice, s));
-        k_gdiag<<<1, 256, 0, s>>>(dv.p + oGD, mmax, mmax, kk, dv.p + oWK);
-        PE_LAUNCH_CHECK();
-        m = kk;
-    }
-
-    // One-root Davidson with the Rayleigh-Ritz eigenproblem on the device, so that
-    // iterations are queued without waiting for the host: iteration i is enqueued,
-    // then the host reads iteration i-1's residual norm and Ritz value from a
-    // host-visible ring slot (written by the kernels) behind that iteration's event
-    // and decides convergence, while the device runs iteration i. The run ends with
-    // the Ritz pair of the last queued iteration (at least as converged as the one
-    // that met the tolerance). Same iteration as run(): fused Ritz pass and
-    // correction while the basis has room, thick restart on the 2 lowest Ritz
-    // vectors otherwise; dependent corrections are excluded on the device (k_gupd).
-    bool run_device(double tol, i64 maxiter, bool fp32, double switch_tol, std::vector<double> &theta, i64 &iters) {
-        const double eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
-        // the device G from the host's
-        for (int i = 0; i < mmax * mmax; ++i) hb.p[oGD + (size_t)i] = 0.0;
-        for (int i = 0; i < m; ++i)
-            for (int j = 0; j < m; ++j) hb.p[oGD + (size_t)(i * mmax + j)] = G[(size_t)i * mmax + j];
-        PE_HIP(hipMemcpyAsync(dv.p + oGD, hb.p + oGD, sizeof(double) * mmax * mmax, hipMemcpyHostToDevice, s));
-        PE_HIP(hipStreamSynchronize(s));  // hb's staging region is reused below
-        struct Slot {
-            hipEvent_t ev = nullptr;
-            int mx = 0;
-            bool restarted = false;
-        };
-        Slot ring[kRing];
-        for (auto &r : ring) PE_HIP(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
-        auto destroy = [&] {
-            for (auto &r : ring)
-                if (r.ev) (void)hipEventDestroy(r.ev);
-        };
-        double best_res = 1e300;
-        int stall = 0;
-        i64 queued = 0, checked = 0;
-        bool stop = false, ok = false;
-        try {
-            while (!stop) {
-                if (queued - checked < 2 && queued < maxiter + 1) {
-                    // ---- queue iteration `queued` ----
-                    Slot &sl = ring[queued % kRing];
-                    double *hslot = hd + oRING + 2 * (queued % kRing);
-                    const bool room = m + 1 <= mmax;
-                    const int kk = room ? 1 : std::max(k, std::min({m, 2 * k, mmax - 1}));
-                    k_rr<<<1, 64, 0, s>>>(dv.p + oGD, mmax, m, kk, dv.p + oCF, dv.p + oYKD, dv.p + oWK, hslot + 1);
-                    PE_LAUNCH_CHECK();
-                    sl.mx = m;
-                    sl.restarted = !room;
-                    const int m0 = room ? m : kk;
-                    if (room) {
-                        ritz_spec_dev();
-                        orthonormalize_device_to(hslot, dv.p + oPAIRD);
-                    } else {
-                        // residual and correction of the Ritz pair of the full basis, then restart
-                        k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, m, dv.p + oCF, dv.p + oCF + 16, 1, dg, 0,
-                                                            X.p, WX.p, T.p, pb.p + pRES);
-                        PE_LAUNCH_CHECK();
-                        k_reduce_partials<<<1, 256, 0, s>>>(pb.p + pRES, nblk, 1, hslot);
-                        PE_LAUNCH_CHECK();
-                        restart_device(kk);
-                        put_correction(0);
-                        orthonormalize_device_to(nullptr, dv.p + oPAIRD);
-                    }
-                    apply(m0, 1, fp32);
-                    dots_to(V.p, m0 + 1, W.p + (i64)m0 * nd, dv.p + oGCOL, sector != 0 && exact);
-                    k_gupd<<<1, 64, 0, s>>>(dv.p + oGD, mmax, m0, dv.p + oGCOL, dv.p + oPAIRD);
-                    PE_LAUNCH_CHECK();
-                    PE_HIP(hipEventRecord(sl.ev, s));
-                    ++queued;
-                    ++st.iterations;
-                    continue;
-                }
-                // ---- check iteration `checked` ----
-                Slot &sl = ring[checked % kRing];
-                PE_HIP(hipEventSynchronize(sl.ev));
-                const double *hslot = hb.p + oRING + 2 * (checked % kRing);
-                const double res = std::sqrt(std::max(hslot[0], 0.0)), th = hslot[1];
-                ++checked;
-                ++iters;
-                const double lim = tol * std::max(std::fabs(th), eps23);
-                if (res <= lim) {
-                    ok = stop = true;
-                } else if (stop_iters >= 0 && iters >= stop_iters && th - res > stop_above) {
-                    stopped_early = ok = stop = true;
-                } else if (fp32 && res <= std::max(lim, switch_tol * std::max(std::fabs(th), 1.0))) {
-                    stop = true;
-                } else if (checked >= maxiter) {
-                    stop = true;
-                } else {
-                    if (res < 0.9 * best_res) {
-                        best_res = res;
-                        stall = 0;
-                    } else {
-                        ++stall;
-                    }
-                    if (fp32 && stall >= 8) stop = true;
-                    if (!fp32 && stall >= 25) {
-                        if (res <= 1e-7 * std::max(std::fabs(th), 1.0)) ok = stop = true;
-                        else if (stall >= 300) stop = true;
-                    }
-                }
-            }
-            // the last queued iteration's Ritz pair
-            PE_HIP(hipStreamSynchronize(s));
-            const i64 last = queued - 1;
-            const Slot &sl = ring[last % kRing];
-            const double *hslot = hb.p + oRING + 2 * (last % kRing);
-            theta.assign(1, hslot[1]);
-            st.residual[0] = std::sqrt(std::max(hslot[0], 0.0));
-            if (sl.restarted) {
-                PE_HIP(hipMemcpyAsync(X.p, V.p, sizeof(double) * nd, hipMemcpyDeviceToDevice, s));
-                PE_HIP(hipMemcpyAsync(WX.p, W.p, sizeof(double) * nd, hipMemcpyDeviceToDevice, s));
-            } else {
-                k_ritz<<<(unsigned)nblk, DT, 0, s>>>(nd, V.p, W.p, nd, sl.mx, dv.p + oCF, dv.p + oCF + 16, 1, dg, 1,
-                                                    X.p, WX.p, T.p, pb.p + pRES);
-                PE_LAUNCH_CHECK();
-            }
-            PE_HIP(hipStreamSynchronize(s));
-        } catch (...) {
-            (void)hipStreamSynchronize(s);
-            destroy();
-            throw;
-        }
-        destroy();
-        // the host copy of the Rayleigh matrix no longer matches the basis
-        std::fill(G.begin(), G.end(), 0.0);
-        return ok;
-    }
-
     // Runs Davidson until every root meets tol (or maxiter iterations). With
     // fp32 set it stops early once residuals reach the FP32 floor (switch_tol).
     // Returns true when converged to tol; X, WX then hold the Ritz pairs.
     // One root: the correction of an iteration is orthogonalised, multiplied and
     // projected before its residual is known, so that an iteration needs a single
     // host round trip (the last iteration's correction is discarded).
-    // Early stop of a one-root run in the second parity sector (stop_iters >= 0):
-    // the run ends, reporting convergence, once it has taken at least stop_iters
-    // iterations (as many as the first sector needed) and its lowest Ritz value
-    // minus its residual norm lies above stop_above (the first sector's converged
-    // root), so that the sector cannot hold the lowest root at the resolution of
-    // this search. stopped_early records that this happened.
-    double stop_above = 0.0;
-    i64 stop_iters = -1;
+    // Early stop of a one-root run in the second parity sector (link set): once the
+    // first sector's run has converged, the run ends, reporting convergence, as soon
+    // as it has taken at least as many iterations as the first and its lowest Ritz
+    // value minus its residual norm lies above the first sector's root, so that the
+    // sector cannot hold the lowest root at the resolution of this search.
+    // stopped_early records that this happened; an error in the first run ends
+    // this one too.
+    const SectorLink *link = nullptr;
+    SectorLink *pub = nullptr;  // the first sector's run publishes its progress here
     bool stopped_early = false;
-    bool device_rr = true;  // one-root runs take run_device (false: the host loop)
+    // The second sector's run does not start iteration i + 1 before the first has
+    // completed i: iterations past the first run's count can only be spent while
+    // the first run is still going, and are wasted if its root wins.
+    void pace(i64 iters) const {
+        if (!link) return;
+        while (link->state.load(std::memory_order_acquire) == 0 &&
+               link->done.load(std::memory_order_acquire) < iters)
+            std::this_thread::yield();
+    }
+    bool stop_now(i64 iters, double theta0, double res0) const {
+        if (!link) return false;
+        const int st_a = link->state.load(std::memory_order_acquire);
+        if (st_a == 3) throw EngineError(PYCI_EHIP, "the other parity sector's run failed");
+        return st_a == 1 && iters >= link->iters && theta0 - res0 > link->theta;
+    }
 
     bool run(double tol, i64 maxiter, bool fp32, double switch_tol, std::vector<double> &theta, i64 &iters) {
-        // one root on an operator whose sigma keeps the sector exactly: the
-        // Rayleigh-Ritz runs on the device and iterations are queued ahead
-        if (k == 1 && exact && mmax <= 17 && m >= 1 && nd > mmax + 1 && device_rr)
-            return run_device(tol, maxiter, fp32, switch_tol, theta, iters);
         const double eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
         std::vector<double> best_res(k, 1e300);
         int stall = 0;
         const bool spec = (k == 1);
         for (;;) {
+            pace(iters);
             std::vector<double> gm((size_t)m * m), w, y;
             for (int i = 0; i < m; ++i)
                 for (int j = 0; j < m; ++j) gm[(size_t)i * m + j] = G[(size_t)i * mmax + j];
@@ -1454,13 +1186,14 @@ struct Davidson {
             }
             ++iters;
             ++st.iterations;
+            if (pub) pub->done.store(iters, std::memory_order_release);
             auto leave = [&](bool ok) {
                 m = m0;  // a speculative column is dropped
                 finish_x(restarted);
                 return ok;
             };
             if (all) return leave(true);
-            if (k == 1 && stop_iters >= 0 && iters >= stop_iters && theta[0] - res[0] > stop_above) {
+            if (k == 1 && stop_now(iters, theta[0], res[0])) {
                 stopped_early = true;
                 return leave(true);
             }
@@ -1542,18 +1275,21 @@ struct Davidson {
 
 // One Davidson run for the k lowest roots (of a parity sector of P when sector
 // != 0). Returns converged flag; theta (k) ascending and the Ritz vectors (k x nd,
-// host) of the run.
+// host) of the run. stream / ws: where the run's kernels go and the matrix-free
+// sigma's workspace (nullptr: the context's); link: see Davidson::link.
 static bool solve_run(pyci_op *op, int k, const double *c0, i64 mm, i64 maxit, double tol, int precision, int sector,
                       std::vector<double> &theta, std::vector<double> &hx, pyci_solve_stats &st,
-                      double stop_above = 0.0, i64 stop_iters = -1, bool *stopped_early = nullptr) {
+                      const SectorLink *link = nullptr, hipStream_t stream = nullptr,
+                      DevBuf<unsigned char> *ws = nullptr, bool *stopped_early = nullptr,
+                      SectorLink *pub = nullptr) {
     const i64 nd = op->nrow;
-    hipStream_t s = op->ctx->stream;
+    hipStream_t s = stream ? stream : op->ctx->stream;
     const i64 nb = op->swap_nb;
     auto partner = [&](i64 J) { return (J % nb) * nb + J / nb; };
-    Davidson D(op, k, (int)mm);
+    Davidson D(op, k, (int)mm, s, ws);
     D.set_sector(sector, nb);
-    D.stop_above = stop_above;
-    D.stop_iters = stop_iters;
+    D.link = link;
+    D.pub = pub;
     std::vector<double> hd((size_t)nd);
     PE_HIP(hipMemcpyAsync(hd.data(), op->diag.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
@@ -1693,16 +1429,41 @@ void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, 
     std::vector<std::pair<double, std::pair<int, int>>> roots;  // (theta, (run, index))
     bool ok_a, ok_b = true;
     if (sectors) {
-        ok_a = solve_run(op, k, c0, mm, maxit, tol, precision, +1, theta_a, hx_a, st_a);
-        // one root: the second sector is searched for as many iterations as the first
-        // needed and then only as long as its lowest Ritz value, less its residual
-        // norm, is not above the first sector's root
+        // The two sectors run concurrently: the second on the context's second stream
+        // and workspace from a host thread of its own, so that each run's kernels
+        // fill the GPU while the other run's host does its Rayleigh-Ritz step. For
+        // one root the second sector's run is searched for as many iterations as the
+        // first needed and then only as long as its lowest Ritz value, less its
+        // residual norm, is not above the first sector's root (SectorLink).
+        pyci_ctx *ctx = op->ctx;
+        if (!ctx->stream2) PE_HIP(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+        // the first sector's work so far (the diagonal, the FP32 values) is visible to the second stream
+        PE_HIP(hipStreamSynchronize(ctx->stream));
+        SectorLink link;
         bool early = false;
-        if (k == 1 && ok_a)
-            ok_b = solve_run(op, k, c0, mm, maxit, tol, precision, -1, theta_b, hx_b, st_b, theta_a[0],
-                             st_a.iterations, &early);
-        else
-            ok_b = solve_run(op, k, c0, mm, maxit, tol, precision, -1, theta_b, hx_b, st_b);
+        std::exception_ptr err_b;
+        std::thread tb([&] {
+            try {
+                PE_HIP(hipSetDevice(ctx->device));
+                ok_b = solve_run(op, k, c0, mm, maxit, tol, precision, -1, theta_b, hx_b, st_b,
+                                 k == 1 ? &link : nullptr, ctx->stream2, &ctx->mf_ws2, &early);
+            } catch (...) {
+                err_b = std::current_exception();
+            }
+        });
+        try {
+            ok_a = solve_run(op, k, c0, mm, maxit, tol, precision, +1, theta_a, hx_a, st_a, nullptr, nullptr,
+                             nullptr, nullptr, k == 1 ? &link : nullptr);
+            link.theta = theta_a[0];
+            link.iters = st_a.iterations;
+            link.state.store(ok_a ? 1 : 2, std::memory_order_release);
+        } catch (...) {
+            link.state.store(3, std::memory_order_release);
+            tb.join();
+            throw;
+        }
+        tb.join();
+        if (err_b) std::rethrow_exception(err_b);
         for (int j = 0; j < k; ++j) {
             roots.push_back({theta_a[(size_t)j], {0, j}});
             roots.push_back({theta_b[(size_t)j], {1, j}});

@@ -739,11 +739,11 @@ void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn) {
 // the context's stream); grown on demand, never cleared (the scatter pass reads
 // only what the product pass of the same sigma wrote).
 template <typename T>
-static T *direct_workspace(const pyci_op *op, i64 elems) {
-    pyci_ctx *ctx = op->ctx;
+static T *direct_workspace(const pyci_op *op, i64 elems, DevBuf<unsigned char> *ws) {
+    DevBuf<unsigned char> &b = ws ? *ws : op->ctx->mf_ws;
     const size_t bytes = sizeof(T) * (size_t)elems;
-    if (ctx->mf_ws.n < bytes) ctx->mf_ws.alloc(bytes);
-    return reinterpret_cast<T *>(ctx->mf_ws.p);
+    if (b.n < bytes) b.alloc(bytes);
+    return reinterpret_cast<T *>(b.p);
 }
 
 // LDS one block may use (160 KB on gfx950) and compute units (256), from the current device.
@@ -759,12 +759,14 @@ template <int MT, typename T, bool SB, int VM, int PE>
 static void launch_dprod_v(const MfView &M, const DprodArgs &A, const T *X, const T *V, const T *kv, T *W, size_t lds,
                            hipStream_t s) {
     auto kern = k_mf_dprod<MT, T, SB, VM, PE>;
-    static bool attr = false;  // dynamic LDS above 64 KB
-    if (!attr) {
+    // dynamic LDS above 64 KB, set once (a function-local static is initialised
+    // once even when two host threads launch the kernel for the first time together)
+    static const bool attr = [&] {
         PE_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds_per_block()));
-        attr = true;
-    }
+        return true;
+    }();
+    (void)attr;
     // persistent grid: the blocks that are resident at once (registers and LDS)
     int per_cu = 0;
     PE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kDprodThreads, lds));
@@ -898,14 +900,15 @@ static MfView view_of(const DirectOp *D) {
 // exchange (only where nup == ndn; the solver's sector runs pass it), which halves
 // the scatter pass (k_mf_symm); 0: any vectors.
 template <typename T>
-void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, hipEvent_t *ev, int sector) {
+void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, hipEvent_t *ev, int sector,
+                  DevBuf<unsigned char> *ws) {
     DirectOp *D = op->direct.get();
     if (D->nup != D->ndn) sector = 0;
     PE_REQUIRE(nvec >= 1 && nvec <= 16, PYCI_EUNSUPPORTED, "the matrix-free operator takes 1..16 vectors");
     const i64 nd = op->nrow;
     const i64 per_vec = ((nd + 63) / 64 * 64) * (i64)D->kpad * (i64)sizeof(T);
     const int chunk = (int)std::max<i64>(1, std::min<i64>(nvec, ((i64)16 << 30) / per_vec));
-    T *W = direct_workspace<T>(op, ((nd * chunk + 63) / 64 * 64) * (i64)D->kpad);
+    T *W = direct_workspace<T>(op, ((nd * chunk + 63) / 64 * 64) * (i64)D->kpad, ws);
     const T *V = std::is_same<T, double>::value ? (const T *)D->V64.p : (const T *)D->V32.p;
     const T *kv = std::is_same<T, double>::value ? (const T *)D->k64.p : (const T *)D->k32.p;
     const MfView M = view_of(D);
@@ -972,8 +975,10 @@ void direct_sigma_noalloc(const pyci_op *op, int nvec, const double *X, double *
                              nullptr, 0);
 }
 
-template void direct_sigma<double>(const pyci_op *, int, const double *, double *, hipStream_t, hipEvent_t *, int);
-template void direct_sigma<float>(const pyci_op *, int, const float *, float *, hipStream_t, hipEvent_t *, int);
+template void direct_sigma<double>(const pyci_op *, int, const double *, double *, hipStream_t, hipEvent_t *, int,
+                                   DevBuf<unsigned char> *);
+template void direct_sigma<float>(const pyci_op *, int, const float *, float *, hipStream_t, hipEvent_t *, int,
+                                  DevBuf<unsigned char> *);
 
 // Per-pass timing of nrep matrix-free sigmas on random X (see pyci_op_bench_sigma_parts).
 // Algorithmic bytes and flops per launch, with N = nd * nvec columns and T the value size:

@@ -130,6 +130,10 @@ struct pyci_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     pe::DevBuf<pe::i64> binom;  // C(n, k) for 0 <= n, k <= 64, row-major [n][k] (65 x 65)
     pe::DevBuf<unsigned char> mf_ws;  // E workspace of the matrix-free sigma (shared by the operators)
+    // second stream and workspace: the solver runs the second parity sector on them,
+    // concurrently with the first (created on first use)
+    hipStream_t stream2 = nullptr;
+    pe::DevBuf<unsigned char> mf_ws2;
     double *noalloc_ws = nullptr;     // the fixed workspace of the allocation-free product on this device
 };
 
@@ -318,8 +322,11 @@ void build_rows(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn, i64 row0,
 // SpMM (sigma.hip): Y = A X, X ncol x nvec row-major, Y nrow x nvec row-major
 // sector = +1 / -1: the vectors lie in that parity sector of the alpha <-> beta
 // exchange (a hint the matrix-free operator uses; 0 = any vectors)
-void sigma_f64(const pyci_op *op, int nvec, const double *X, double *Y, hipStream_t s, int sector = 0);
-void sigma_f32(const pyci_op *op, int nvec, const float *X, float *Y, hipStream_t s, int sector = 0);
+// ws (optional): the matrix-free operator's E workspace (default: the context's mf_ws)
+void sigma_f64(const pyci_op *op, int nvec, const double *X, double *Y, hipStream_t s, int sector = 0,
+               DevBuf<unsigned char> *ws = nullptr);
+void sigma_f32(const pyci_op *op, int nvec, const float *X, float *Y, hipStream_t s, int sector = 0,
+               DevBuf<unsigned char> *ws = nullptr);
 void ensure_vals32(pyci_op *op);
 // Davidson (davidson.hip)
 void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, double tol, int precision,
@@ -363,7 +370,7 @@ bool direct_applicable(const pyci_ham *ham, const pyci_wfn *wfn, i64 nrow, i64 n
 void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn);
 template <typename T>
 void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, hipEvent_t *ev = nullptr,
-                  int sector = 0);
+                  int sector = 0, DevBuf<unsigned char> *ws = nullptr);
 void direct_bench_parts(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *ms, double *bytes, double *flops);
 void direct_ensure_csr(pyci_op *op);
 // Allocation-free matrix-free product (direct.hip): E in a fixed per-device array

@@ -214,13 +214,15 @@ static void spmm(const pyci_op *op, const VT *vals, int nvec, const XT *X, XT *Y
     PE_LAUNCH_CHECK();
 }
 
-void sigma_f64(const pyci_op *op, int nvec, const double *X, double *Y, hipStream_t s, int sector) {
-    if (op->storage == PYCI_STORAGE_DIRECT) return direct_sigma<double>(op, nvec, X, Y, s, nullptr, sector);
+void sigma_f64(const pyci_op *op, int nvec, const double *X, double *Y, hipStream_t s, int sector,
+               DevBuf<unsigned char> *ws) {
+    if (op->storage == PYCI_STORAGE_DIRECT) return direct_sigma<double>(op, nvec, X, Y, s, nullptr, sector, ws);
     spmm<double, double>(op, op->vals.p, nvec, X, Y, s);
 }
 
-void sigma_f32(const pyci_op *op, int nvec, const float *X, float *Y, hipStream_t s, int sector) {
-    if (op->storage == PYCI_STORAGE_DIRECT) return direct_sigma<float>(op, nvec, X, Y, s, nullptr, sector);
+void sigma_f32(const pyci_op *op, int nvec, const float *X, float *Y, hipStream_t s, int sector,
+               DevBuf<unsigned char> *ws) {
+    if (op->storage == PYCI_STORAGE_DIRECT) return direct_sigma<float>(op, nvec, X, Y, s, nullptr, sector, ws);
     spmm<float, float>(op, op->vals32.p, nvec, X, Y, s);
 }
 
