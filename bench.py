@@ -378,9 +378,33 @@ def cpu_baseline(wl, one, two, dets, sigma_calls, budget_s):
     t_spmv_rows = (time.perf_counter() - t0) / reps
     nnz_sample = L.oracle_csr_nnz(h)
     L.oracle_csr_free(h)
-    total_lower = None
+    # the same rows built on every host thread of this GPU's share (row ranges per
+    # thread); the threaded product is not sampled this way (a contiguous row sample
+    # lands on one thread of a row-partitioned product), its full-space timing is
+    # the nbasis 8 leg
+    from concurrent.futures import ThreadPoolExecutor
+
+    nth = host_threads()
+    step = (rows + nth - 1) // nth
+    ranges = [(r0 + a, min(r0 + rows, r0 + a + step)) for a in range(0, rows, step)]
+
+    def part(rr):
+        O.csr_build("fullci", wl["nbasis"], wl["nup"], wl["ndn"], dets, one, two, nrow=rr[1], symmetric=True,
+                    row0=rr[0])
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=nth) as ex:
+        list(ex.map(part, ranges))
+    t_build_mt = time.perf_counter() - t0
+    # every thread rebuilt the determinant index: count it once, as a full build would
+    t_row_mt = max(t_build_mt - t_index, 1e-9) / rows
     return dict(t_row=t_row, t_index=t_index, rows=rows, t_spmv_sample=t_spmv_rows, nnz_sample=nnz_sample,
-                sigma_calls=sigma_calls, total_lower=total_lower)
+                sigma_calls=sigma_calls, nthreads=nth, t_row_mt=t_row_mt)
+
+
+def host_threads():
+    """Host threads of one GPU's share of the box (16), or fewer where fewer cores exist."""
+    return max(1, min(os.cpu_count() or 1, 16))
 
 
 def main():
@@ -395,6 +419,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nvec", type=int, default=1, help="vectors per sigma in --sigma-only runs")
     ap.add_argument("--sigma-only", action="store_true",
                     help="build the operator, run a few sigma launches and a device copy of known size "
                          "(for hardware-counter runs), print their timings and exit")
@@ -470,10 +495,10 @@ def main():
     if args.sigma_only:
         op = build()
         nrow, ncol, size_ref, nnz = info(op)
-        pr = parts(op, 1, 4, 1 if args.precision == "mixed" else 0)
+        pr = parts(op, args.nvec, 4, 1 if args.precision == "mixed" else 0)
         copy_bytes = 4 << 30
         t_copy = device_copy(copy_bytes)
-        print(json.dumps({"storage": storage_of(op), "parts": pr, "nnz": nnz, "copy_bytes": copy_bytes,
+        print(json.dumps({"storage": storage_of(op), "nvec": args.nvec, "parts": pr, "nnz": nnz, "copy_bytes": copy_bytes,
                           "copy_ms": t_copy * 1e3}), flush=True)
         return
     # FP64 reference energies (untimed, tight tolerance): from the CSR operator,
@@ -494,7 +519,7 @@ def main():
                 lanczos_products = int(g["lanczos_products_k1"])
         e_src = "CPU oracle (tests/golden/data/%s.npz: oracle CSR + ARPACK, tol 1e-12)" % wl["name"]
     elif wl["name"] == "fci8" and not args.no_cpu:
-        cpu8 = {n: cpu_full_fci8(n) for n in (1, os.cpu_count() or 1)}
+        cpu8 = {n: cpu_full_fci8(n) for n in (1, host_threads())}
         e_ref = np.array([cpu8[1][2]])
         e_src = "CPU oracle solve timed in cpu_baseline (oracle CSR + ARPACK, tol 1e-12)"
     else:
@@ -558,6 +583,17 @@ def main():
     ok(L, L.pyci_ctx_synchronize(ctx), "sync")
     t_all = time.perf_counter() - t_all0
     tts = t_all / args.steps
+    # the other precision of the same operator, timed alike (reported beside the line)
+    other = None
+    if not args.sigma_only:
+        po = 1 - prec
+        step(args.storage, po)
+        ores = [step(args.storage, po) for _ in range(max(1, args.steps))]
+        other = {"precision": "mixed" if po == 1 else "f64",
+                 "tts_s": round(float(np.mean([r[0] + r[1] for r in ores])), 6),
+                 "max_abs_dE": float(max(np.max(np.abs(np.sort(r[2]) - e_ref)) for r in ores)),
+                 "iterations": int(ores[0][3].iterations)}
+        log(f"{other['precision']}: {other['tts_s']*1e3:.1f} ms per step, |dE| {other['max_abs_dE']:.2e}")
     tb = float(np.mean([r[0] for r in res]))
     ts = float(np.mean([r[1] for r in res]))
     max_de = max(r[2] for r in res)
@@ -621,6 +657,7 @@ def main():
         "iterations": int(np.mean([r[3] for r in res])),
         "sigma_calls": sigma_calls,
         "sigma_parts_ms": {nm: round(v["ms"], 4) for nm, v in zip(names, tparts)},
+        "other_precision": other,
         "roofline": roofline,
     }
     if has_csr:
@@ -658,15 +695,17 @@ def main():
             "extrapolated": True,
             "sample_fraction": round(cb["rows"] / nd, 4),
         }
+        out["cpu_baseline"][f"dets_per_s_built_threads_{cb['nthreads']}"] = round(1.0 / cb["t_row_mt"], 1)
         if cpu8 is None:
-            cpu8 = {n: cpu_full_fci8(n) for n in (1, os.cpu_count() or 1)}
+            cpu8 = {n: cpu_full_fci8(n) for n in (1, host_threads())}
         out["cpu_baseline"]["fci8_full"] = {
             "workload": "FullCI nbasis=8 4a+4b seeded integrals (fci8), 4,900 determinants, nroot=1, timed in full",
             **{f"threads_{n}": {"build_s": round(v[0], 4), "solve_s": round(v[1], 4), "total_s": round(v[0] + v[1], 4),
                                 "products": v[3], "energy": v[2]} for n, v in cpu8.items()},
             "cores_available": os.cpu_count(),
         }
-        log(f"cpu baseline estimate {t_cpu:.1f} s; fci8 in full: " +
+        log(f"cpu baseline estimate {t_cpu:.1f} s (1 thread; build {1.0 / cb['t_row']:.0f} dets/s, on "
+            f"{cb['nthreads']} threads {1.0 / cb['t_row_mt']:.0f} dets/s); fci8 in full: " +
             ", ".join(f"{n} threads {v[0] + v[1]:.3f} s" for n, v in cpu8.items()))
     print(json.dumps(out), flush=True)
 
