@@ -66,7 +66,7 @@ WORKLOADS = {
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("graph_launches", ctypes.c_int64), ("rows_per_block", ctypes.c_int64), ("converged", ctypes.c_int64),
-                ("setup_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+                ("setup_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("single_path", ctypes.c_int64)]
 
 
 def bench_batch(args, wl):
@@ -143,7 +143,8 @@ def bench_batch(args, wl):
         de = float(np.max(np.abs(ev - e_ref)))
         res.append((t, ev, lat, it, ns, cv, st, de))
         log(f"step {k}: {t*1e3:.1f} ms ({st.setup_ms:.1f} ms set-up), {int(cv.sum())}/{npb} converged, "
-            f"max |dE| {de:.2e}, {st.graph_launches} iterations, rows/block {st.rows_per_block}")
+            f"max |dE| {de:.2e}, {st.graph_launches} graph iterations, rows/block {st.rows_per_block}, "
+            f"{st.single_path} problems on the single-problem solver")
     tt = float(np.mean([r[0] for r in res]))
     lat = np.concatenate([r[2] for r in res])
     dims = [comb(q["nbasis"], q["nocc"]) ** 2 for q in probs]
@@ -167,6 +168,7 @@ def bench_batch(args, wl):
         "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
         "setup_ms": round(float(np.mean([r[6].setup_ms for r in res])), 3),
         "iterations_max": int(max(r[6].graph_launches for r in res)),
+        "single_path": int(res[-1][6].single_path),
         "rows_per_block": int(res[-1][6].rows_per_block),
         "converged": int(min(r[5].sum() for r in res)),
         "max_abs_dE": max(r[7] for r in res),

@@ -284,16 +284,19 @@ int pyci_fanci_jacobian(pyci_fanci *f, const double *x, double *y);
 /* ------------------------------------------------------------------------
  * Batched solves of many uneven FullCI problems. No counterpart in the
  * reference: each problem is SparseOp(ham, wfn) + solve_ci(n) (sparseop.cpp:
- * 114-146) on the full FullCI space of its own integrals (add_all_dets), and all
- * problems run together on the device (matrix-free sigma, one kernel launch per
- * Davidson step over all problems, the iteration replayed as a hipGraph).
+ * 114-146) on the full FullCI space of its own integrals (add_all_dets). The
+ * problems below 65,536 determinants run together on the device (matrix-free
+ * sigma, one kernel launch per Davidson step over all of them, the iteration
+ * replayed as a hipGraph); the larger ones, each of which fills the GPU by
+ * itself, are solved one after another by the single-problem solver beside them.
  * ------------------------------------------------------------------------ */
 typedef struct pyci_batch_stats {
-    int64_t graph_launches; /* Davidson iterations replayed (the batch's maximum) */
+    int64_t graph_launches; /* Davidson iterations replayed (the graph's maximum) */
     int64_t rows_per_block; /* autotuned rows per workgroup of the tall-skinny kernels */
     int64_t converged;      /* problems that converged */
-    double setup_ms;        /* host set-up (tables, integrals, diagonals) to the first iteration */
+    double setup_ms;        /* host set-up of the graph path (tables, integrals, diagonals) */
     double total_ms;        /* whole call */
+    int64_t single_path;    /* problems solved by the single-problem solver */
 } pyci_batch_stats;
 
 /* nprob problems: nbasis[p] <= 15 spatial orbitals, nocc_up[p] >= nocc_dn[p] >= 1,
@@ -301,8 +304,10 @@ typedef struct pyci_batch_stats {
  * pyci_ham_create. nroot <= 8 lowest roots each, tol as pyci_op_solve, precision
  * PYCI_PREC_F64 or PYCI_PREC_MIXED. Outputs (host, may be NULL except evals):
  * evals[p * nroot + j] (ecore added; descending for nroot > 1 as solve_ci),
- * latency_ms[p] from the first iteration to the problem's convergence (device
- * clock), iterations[p], nsigma[p] (vectors multiplied by H), converged[p]. */
+ * latency_ms[p] from the first iteration of the problem's path to its
+ * convergence (device clock), iterations[p], nsigma[p] (vectors multiplied by H),
+ * converged[p] (an unconverged problem of the single-problem path reports NaN
+ * roots). */
 int pyci_batch_solve(pyci_ctx *ctx, int64_t nprob, const int64_t *nbasis, const int64_t *nocc_up,
                      const int64_t *nocc_dn, const double *ecore, const double *const *one_mo,
                      const double *const *two_mo, int64_t nroot, double tol, int precision, int64_t maxiter,

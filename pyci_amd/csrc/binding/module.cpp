@@ -625,6 +625,7 @@ PYBIND11_MODULE(_pyci, m) {
             info["rows_per_block"] = st.rows_per_block;
             info["setup_ms"] = st.setup_ms;
             info["total_ms"] = st.total_ms;
+            info["single_path"] = st.single_path;
             return py::make_tuple(ev, info);
         },
         py::arg("problems"), py::arg("n") = 2, py::arg("tol") = 1.0e-6, py::arg("precision") = "mixed",

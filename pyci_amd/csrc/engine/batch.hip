@@ -23,7 +23,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <limits>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"
@@ -756,6 +759,7 @@ __global__ void __launch_bounds__(256) k_binit(BView b, const i64 *sa, const i64
 // ---------------------------------------------------------------------------
 struct Batch {
     pyci_ctx *ctx = nullptr;
+    hipStream_t s = nullptr;  // the stream the batch's work and graph go to
     int nprob = 0, k = 0, kb0 = 0, mmax = 0, S = 0, RB = 1024;
     std::vector<BProb> hp;
     DevBuf<BProb> prob;
@@ -814,7 +818,7 @@ static void build_row_map(Batch &B, int RB) {
     B.nrowblk = (i64)blk.size();
     B.row_blk.alloc(blk.size());
     B.row_first.alloc(first.size());
-    hipStream_t s = B.ctx->stream;
+    hipStream_t s = B.s;
     PE_HIP(hipMemcpyAsync(B.row_blk.p, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, s));
     PE_HIP(hipMemcpyAsync(B.row_first.p, first.data(), sizeof(i64) * first.size(), hipMemcpyHostToDevice, s));
     B.partial.alloc((size_t)B.nrowblk * B.S);
@@ -919,16 +923,15 @@ static void iteration(Batch &B, hipStream_t s) {
     PE_LAUNCH_CHECK();
 }
 
-void batch_solve(pyci_ctx *ctx, i64 nprob, const i64 *nbasis, const i64 *nup, const i64 *ndn, const double *ecore,
-                 const double *const *one_mo, const double *const *two_mo, i64 nroot, double tol, int precision,
-                 i64 maxiter, double *evals, double *latency_ms, i64 *iters, i64 *nsigma, int *converged,
-                 pyci_batch_stats *stats) {
-    PE_REQUIRE(nprob >= 1, PYCI_EINVAL, "empty batch");
-    PE_REQUIRE(nroot >= 1 && nroot <= BK_MAX, PYCI_EUNSUPPORTED, "batched solves take 1..8 roots");
-    hipStream_t s = ctx->stream;
+// The replayed-graph path over all given problems, on stream s.
+static void batch_graph(pyci_ctx *ctx, hipStream_t s, i64 nprob, const i64 *nbasis, const i64 *nup, const i64 *ndn,
+                        const double *ecore, const double *const *one_mo, const double *const *two_mo, i64 nroot,
+                        double tol, int precision, i64 maxiter, double *evals, double *latency_ms, i64 *iters,
+                        i64 *nsigma, int *converged, pyci_batch_stats *stats) {
     const auto t_start = std::chrono::steady_clock::now();
     Batch B;
     B.ctx = ctx;
+    B.s = s;
     B.k = (int)nroot;
     B.mmax = (int)std::max<i64>({2 * nroot + 1, 10, 6 * nroot});
     PE_REQUIRE(B.mmax <= BM_MAX, PYCI_EUNSUPPORTED, "subspace too large");
@@ -1270,22 +1273,32 @@ void batch_solve(pyci_ctx *ctx, i64 nprob, const i64 *nbasis, const i64 *nup, co
         k_binit<<<(unsigned)B.nrowblk, 256, 0, s>>>(B.view, dsa.p, dsb.p, dca.p, dcb.p);
         PE_LAUNCH_CHECK();
         float best_ms = 1e30f;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        PE_HIP(hipEventCreate(&e0));
+        struct EvGuard {
+            hipEvent_t &a, &b;
+            ~EvGuard() {
+                if (a) (void)hipEventDestroy(a);
+                if (b) (void)hipEventDestroy(b);
+            }
+        } evg{e0, e1};
+        PE_HIP(hipEventCreate(&e1));
         for (int rb : {256, 1024, 4096}) {
             build_row_map(B, rb);
             const unsigned gy_full = (unsigned)(((B.mmax + 7) / 8) * ((B.kb0 + 7) / 8));
             k_bgram<<<dim3((unsigned)B.nrowblk, gy_full), 256, 0, s>>>(B.view, 2);  // warm-up
             PE_LAUNCH_CHECK();
-            PE_HIP(hipEventRecord(ctx->ev0, s));
+            PE_HIP(hipEventRecord(e0, s));
             for (int r = 0; r < 3; ++r) {
                 k_bgram<<<dim3((unsigned)B.nrowblk, gy_full), 256, 0, s>>>(B.view, 2);
                 PE_LAUNCH_CHECK();
                 k_britz<<<(unsigned)B.nrowblk, 256, 0, s>>>(B.view);
                 PE_LAUNCH_CHECK();
             }
-            PE_HIP(hipEventRecord(ctx->ev1, s));
-            PE_HIP(hipEventSynchronize(ctx->ev1));
+            PE_HIP(hipEventRecord(e1, s));
+            PE_HIP(hipEventSynchronize(e1));
             float ms = 0.f;
-            PE_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            PE_HIP(hipEventElapsedTime(&ms, e0, e1));
             if (ms < best_ms) {
                 best_ms = ms;
                 best_rb = rb;
@@ -1382,6 +1395,161 @@ void batch_solve(pyci_ctx *ctx, i64 nprob, const i64 *nbasis, const i64 *nup, co
         stats->total_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
         stats->setup_ms = std::chrono::duration<double, std::milli>(t_setup - t_start).count();
         stats->converged = nconv;
+    }
+}
+
+// One problem by the single-problem solver: the full FullCI space (add_all_dets),
+// the matrix-free operator, and the Davidson solve of davidson.hip with its parity
+// sectors on their own streams. A run that does not converge reports NaN roots and
+// converged = 0, as the graph path reports its unconverged problems.
+static void single_solve(pyci_ctx *ctx, int n, int na, int nb, double ecore, const double *one, const double *two,
+                         i64 nroot, double tol, int precision, i64 maxiter, double *ev, i64 *it, i64 *ns, int *conv) {
+    auto check = [](int rc) {
+        if (rc != PYCI_OK) throw EngineError(rc, pyci_last_error());
+    };
+    const i64 nd = binom_host(n, na) * binom_host(n, nb);
+    std::vector<uint64_t> dets((size_t)(2 * nd));
+    check(pyci_fill_all_dets(ctx, PYCI_WFN_FULLCI, n, na, nb, dets.data()));
+    pyci_ham *h = nullptr;
+    pyci_wfn *w = nullptr;
+    pyci_op *op = nullptr;
+    struct Guard {
+        pyci_ham *&h;
+        pyci_wfn *&w;
+        pyci_op *&op;
+        ~Guard() {
+            pyci_op_destroy(op);
+            pyci_wfn_destroy(w);
+            pyci_ham_destroy(h);
+        }
+    } guard{h, w, op};
+    check(pyci_ham_create(ctx, n, ecore, one, two, &h));
+    check(pyci_wfn_create(ctx, PYCI_WFN_FULLCI, n, na, nb, nd, dets.data(), &w));
+    check(pyci_op_create_ex(h, w, -1, -1, 1, PYCI_STORAGE_AUTO, &op));
+    std::vector<double> vecs((size_t)(nroot * nd));
+    pyci_solve_stats st{};
+    const int rc = pyci_op_solve(op, nroot, nullptr, -1, maxiter, tol, precision, ev, vecs.data(), &st);
+    if (rc == PYCI_ENOCONV) {
+        for (i64 j = 0; j < nroot; ++j) ev[j] = std::numeric_limits<double>::quiet_NaN();
+        *conv = 0;
+    } else {
+        check(rc);
+        *conv = 1;
+    }
+    *it = st.iterations;
+    *ns = st.sigma_calls;
+}
+
+// Problems of at least kSingleND determinants (on the device a single problem of
+// that size fills the GPU by itself) take the single-problem solver, one after
+// another from the largest down on a host thread of their own, while the smaller
+// ones run together as the replayed graph on a stream of their own. The graph's
+// iteration costs the sum of every active problem's work, so a large problem in it
+// sets the pace for all; alone, it runs at the single solver's speed (FullCI nbasis
+// 12: 0.10-0.18 s for two roots in the single solver, 1.4-2.0 s of latency inside
+// the 64-problem graph).
+constexpr i64 kSingleND = 65536;
+
+void batch_solve(pyci_ctx *ctx, i64 nprob, const i64 *nbasis, const i64 *nup, const i64 *ndn, const double *ecore,
+                 const double *const *one_mo, const double *const *two_mo, i64 nroot, double tol, int precision,
+                 i64 maxiter, double *evals, double *latency_ms, i64 *iters, i64 *nsigma, int *converged,
+                 pyci_batch_stats *stats) {
+    PE_REQUIRE(nprob >= 1, PYCI_EINVAL, "empty batch");
+    PE_REQUIRE(nroot >= 1 && nroot <= BK_MAX, PYCI_EUNSUPPORTED, "batched solves take 1..8 roots");
+    for (i64 p = 0; p < nprob; ++p)
+        PE_REQUIRE(nbasis[p] >= 2 && nbasis[p] <= 15 && ndn[p] >= 1 && ndn[p] <= nup[p] && nup[p] <= nbasis[p],
+                   PYCI_EUNSUPPORTED, "batched problems need 2 <= nbasis <= 15 and 1 <= ndn <= nup <= nbasis");
+    const auto t_start = std::chrono::steady_clock::now();
+    std::vector<i64> big, small;
+    for (i64 p = 0; p < nprob; ++p) {
+        const i64 nd = binom_host((int)nbasis[p], (int)nup[p]) * binom_host((int)nbasis[p], (int)ndn[p]);
+        (nd >= kSingleND ? big : small).push_back(p);
+    }
+    std::stable_sort(big.begin(), big.end(), [&](i64 a, i64 b) {
+        return binom_host((int)nbasis[a], (int)nup[a]) * binom_host((int)nbasis[a], (int)ndn[a]) >
+               binom_host((int)nbasis[b], (int)nup[b]) * binom_host((int)nbasis[b], (int)ndn[b]);
+    });
+    if (big.empty()) {
+        batch_graph(ctx, ctx->stream, nprob, nbasis, nup, ndn, ecore, one_mo, two_mo, nroot, tol, precision, maxiter,
+                    evals, latency_ms, iters, nsigma, converged, stats);
+        if (stats) stats->single_path = 0;
+        return;
+    }
+    // the small problems' arguments and outputs, compacted
+    const i64 ns_ = (i64)small.size();
+    std::vector<i64> snb(ns_), snu(ns_), snd(ns_), sit(ns_), sns(ns_);
+    std::vector<double> sec(ns_), sev((size_t)(ns_ * nroot)), slat(ns_);
+    std::vector<const double *> sone(ns_), stwo(ns_);
+    std::vector<int> sconv(ns_);
+    for (i64 i = 0; i < ns_; ++i) {
+        const i64 p = small[(size_t)i];
+        snb[i] = nbasis[p], snu[i] = nup[p], snd[i] = ndn[p], sec[i] = ecore[p];
+        sone[i] = one_mo[p], stwo[i] = two_mo[p];
+    }
+    std::vector<double> bev((size_t)(big.size() * nroot)), blat(big.size());
+    std::vector<i64> bit(big.size()), bns(big.size());
+    std::vector<int> bconv(big.size());
+    const int device = ctx->device;
+    std::exception_ptr err_big;
+    std::thread tb([&] {
+        try {
+            PE_HIP(hipSetDevice(device));
+            int rate_khz = 0;
+            PE_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device));
+            DevBuf<unsigned long long> stamp(1);
+            auto now = [&] {
+                unsigned long long t = 0;
+                mf_wallclock_launch(stamp.p, ctx->stream);
+                PE_HIP(hipMemcpyAsync(&t, stamp.p, sizeof(t), hipMemcpyDeviceToHost, ctx->stream));
+                PE_HIP(hipStreamSynchronize(ctx->stream));
+                return t;
+            };
+            const unsigned long long t0 = now();
+            for (size_t i = 0; i < big.size(); ++i) {
+                const i64 p = big[i];
+                single_solve(ctx, (int)nbasis[p], (int)nup[p], (int)ndn[p], ecore[p], one_mo[p], two_mo[p], nroot,
+                             tol, precision, maxiter, &bev[i * (size_t)nroot], &bit[i], &bns[i], &bconv[i]);
+                blat[i] = rate_khz > 0 ? (double)(now() - t0) / (double)rate_khz : -1.0;
+            }
+        } catch (...) {
+            err_big = std::current_exception();
+        }
+    });
+    hipStream_t gs = nullptr;
+    pyci_batch_stats gst{};
+    try {
+        if (ns_ > 0) {
+            PE_HIP(hipStreamCreateWithFlags(&gs, hipStreamNonBlocking));
+            batch_graph(ctx, gs, ns_, snb.data(), snu.data(), snd.data(), sec.data(), sone.data(), stwo.data(), nroot,
+                        tol, precision, maxiter, sev.data(), slat.data(), sit.data(), sns.data(), sconv.data(), &gst);
+        }
+    } catch (...) {
+        tb.join();
+        if (gs) (void)hipStreamDestroy(gs);
+        throw;
+    }
+    tb.join();
+    if (gs) PE_HIP(hipStreamDestroy(gs));
+    if (err_big) std::rethrow_exception(err_big);
+    i64 nconv = 0;
+    auto put = [&](i64 p, const double *ev, double lat, i64 it, i64 nsg, int cv) {
+        for (i64 j = 0; j < nroot; ++j) evals[p * nroot + j] = ev[j];
+        if (latency_ms) latency_ms[p] = lat;
+        if (iters) iters[p] = it;
+        if (nsigma) nsigma[p] = nsg;
+        if (converged) converged[p] = cv;
+        nconv += cv ? 1 : 0;
+    };
+    for (i64 i = 0; i < ns_; ++i)
+        put(small[(size_t)i], &sev[(size_t)(i * nroot)], slat[(size_t)i], sit[(size_t)i], sns[(size_t)i],
+            sconv[(size_t)i]);
+    for (size_t i = 0; i < big.size(); ++i)
+        put(big[i], &bev[i * (size_t)nroot], blat[i], bit[i], bns[i], bconv[i]);
+    if (stats) {
+        *stats = gst;
+        stats->converged = nconv;
+        stats->single_path = (i64)big.size();
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     }
 }
 

@@ -590,11 +590,11 @@ static void jacobi_eig(int m, std::vector<double> a, std::vector<double> &w, std
     v.swap(vv);
 }
 
-// What the first parity sector's one-root run tells the second one, which runs
+// What the first parity sector's run tells the second one, which runs
 // concurrently on another stream and host thread: the first run's state (0
 // running, 1 converged, 2 ended unconverged, 3 aborted by an error), its
-// completed iterations while it runs, and once it has converged, its root and
-// iteration count.
+// completed iterations while it runs, and once it has converged, its highest
+// requested root (theta[k - 1]) and iteration count.
 struct SectorLink {
     std::atomic<int> state{0};
     std::atomic<i64> done{0};
@@ -1113,13 +1113,14 @@ struct Davidson {
     // One root: the correction of an iteration is orthogonalised, multiplied and
     // projected before its residual is known, so that an iteration needs a single
     // host round trip (the last iteration's correction is discarded).
-    // Early stop of a one-root run in the second parity sector (link set): once the
-    // first sector's run has converged, the run ends, reporting convergence, as soon
-    // as it has taken at least as many iterations as the first and its lowest Ritz
-    // value minus its residual norm lies above the first sector's root, so that the
-    // sector cannot hold the lowest root at the resolution of this search.
-    // stopped_early records that this happened; an error in the first run ends
-    // this one too.
+    // Early stop of the second parity sector's run (link set): once the first
+    // sector's run has converged, the run ends, reporting convergence, as soon as it
+    // has taken at least as many iterations as the first and each of its Ritz pairs
+    // has either converged (in the FP64 phase) or a Ritz value minus residual norm
+    // above the first sector's highest root, so that no unconverged pair of this
+    // sector can be among the k lowest roots of the two at the resolution of this
+    // search. stopped_early records that this happened; an error in the first run
+    // ends this one too.
     const SectorLink *link = nullptr;
     SectorLink *pub = nullptr;  // the first sector's run publishes its progress here
     bool stopped_early = false;
@@ -1132,11 +1133,20 @@ struct Davidson {
                link->done.load(std::memory_order_acquire) < iters)
             std::this_thread::yield();
     }
-    bool stop_now(i64 iters, double theta0, double res0) const {
+    bool stop_now(i64 iters, const std::vector<double> &theta, const std::vector<double> &res, double tol,
+                  bool fp32) const {
         if (!link) return false;
         const int st_a = link->state.load(std::memory_order_acquire);
         if (st_a == 3) throw EngineError(PYCI_EHIP, "the other parity sector's run failed");
-        return st_a == 1 && iters >= link->iters && theta0 - res0 > link->theta;
+        if (st_a != 1 || iters < link->iters) return false;
+        const double eps23 = std::pow(2.220446049250313e-16, 2.0 / 3.0);
+        for (int j = 0; j < k; ++j) {
+            const bool above = theta[(size_t)j] - res[(size_t)j] > link->theta;
+            // a root of this sector below the first's is kept only with its FP64 value
+            const bool conv = !fp32 && res[(size_t)j] <= tol * std::max(std::fabs(theta[(size_t)j]), eps23);
+            if (!above && !conv) return false;
+        }
+        return true;
     }
 
     bool run(double tol, i64 maxiter, bool fp32, double switch_tol, std::vector<double> &theta, i64 &iters) {
@@ -1193,7 +1203,7 @@ struct Davidson {
                 return ok;
             };
             if (all) return leave(true);
-            if (k == 1 && stop_now(iters, theta[0], res[0])) {
+            if (stop_now(iters, theta, res, tol, fp32)) {
                 stopped_early = true;
                 return leave(true);
             }
@@ -1431,10 +1441,10 @@ void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, 
     if (sectors) {
         // The two sectors run concurrently: the second on the context's second stream
         // and workspace from a host thread of its own, so that each run's kernels
-        // fill the GPU while the other run's host does its Rayleigh-Ritz step. For
-        // one root the second sector's run is searched for as many iterations as the
-        // first needed and then only as long as its lowest Ritz value, less its
-        // residual norm, is not above the first sector's root (SectorLink).
+        // fill the GPU while the other run's host does its Rayleigh-Ritz step. The
+        // second sector's run is searched for as many iterations as the first needed
+        // and then only as long as one of its Ritz pairs could still be among the k
+        // lowest roots (SectorLink, Davidson::stop_now).
         pyci_ctx *ctx = op->ctx;
         if (!ctx->stream2) PE_HIP(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
         // the first sector's work so far (the diagonal, the FP32 values) is visible to the second stream
@@ -1445,16 +1455,16 @@ void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, 
         std::thread tb([&] {
             try {
                 PE_HIP(hipSetDevice(ctx->device));
-                ok_b = solve_run(op, k, c0, mm, maxit, tol, precision, -1, theta_b, hx_b, st_b,
-                                 k == 1 ? &link : nullptr, ctx->stream2, &ctx->mf_ws2, &early);
+                ok_b = solve_run(op, k, c0, mm, maxit, tol, precision, -1, theta_b, hx_b, st_b, &link,
+                                 ctx->stream2, &ctx->mf_ws2, &early);
             } catch (...) {
                 err_b = std::current_exception();
             }
         });
         try {
             ok_a = solve_run(op, k, c0, mm, maxit, tol, precision, +1, theta_a, hx_a, st_a, nullptr, nullptr,
-                             nullptr, nullptr, k == 1 ? &link : nullptr);
-            link.theta = theta_a[0];
+                             nullptr, nullptr, &link);
+            link.theta = theta_a[(size_t)k - 1];
             link.iters = st_a.iterations;
             link.state.store(ok_a ? 1 : 2, std::memory_order_release);
         } catch (...) {

@@ -11,7 +11,11 @@
   * The 64-problem uneven batch (nbasis 8..12, 3..6 electrons of each spin):
     every root of one batch_solve call, FP64 and mixed, against
     tests/golden/data/batch64.npz (the oracle's CSR + ARPACK for every problem).
-Reference path: sparseop.cpp:96-146 (perform_op, solve_ci), rdm.cpp:269-530.
+  * GenCI over 48 spin-orbitals with 12 electrons (24 spatial orbitals, 64
+    decaying vectors, seed 1003): HF + singles + doubles, two heat-bath steps and
+    the learned selector's step, against tests/golden/data/genci24.npz.
+Reference path: sparseop.cpp:96-146 (perform_op, solve_ci), rdm.cpp:269-530,
+hci.cpp:184-279.
 """
 import hashlib
 
