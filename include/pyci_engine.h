@@ -285,7 +285,7 @@ int pyci_fanci_jacobian(pyci_fanci *f, const double *x, double *y);
  * Batched solves of many uneven FullCI problems. No counterpart in the
  * reference: each problem is SparseOp(ham, wfn) + solve_ci(n) (sparseop.cpp:
  * 114-146) on the full FullCI space of its own integrals (add_all_dets). The
- * problems below 65,536 determinants run together on the device (matrix-free
+ * problems below 150,000 determinants run together on the device (matrix-free
  * sigma, one kernel launch per Davidson step over all of them, the iteration
  * replayed as a hipGraph); the larger ones, each of which fills the GPU by
  * itself, are solved one after another by the single-problem solver beside them.

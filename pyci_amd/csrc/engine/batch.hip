@@ -1440,15 +1440,17 @@ static void single_solve(pyci_ctx *ctx, int n, int na, int nb, double ecore, con
     *ns = st.sigma_calls;
 }
 
-// Problems of at least kSingleND determinants (on the device a single problem of
-// that size fills the GPU by itself) take the single-problem solver, one after
-// another from the largest down on a host thread of their own, while the smaller
-// ones run together as the replayed graph on a stream of their own. The graph's
-// iteration costs the sum of every active problem's work, so a large problem in it
-// sets the pace for all; alone, it runs at the single solver's speed (FullCI nbasis
-// 12: 0.10-0.18 s for two roots in the single solver, 1.4-2.0 s of latency inside
-// the 64-problem graph).
-constexpr i64 kSingleND = 65536;
+// Problems of at least kSingleND determinants take the single-problem solver, one
+// after another from the largest down on a host thread of their own, while the
+// smaller ones run together as the replayed graph on a stream of their own. The
+// graph's iteration costs the sum of every active problem's work, so a large
+// problem in it sets the pace for all; alone, it runs at the single solver's speed
+// (FullCI nbasis 12: 0.10-0.18 s for two roots in the single solver, 1.4-2.0 s of
+// latency inside the 64-problem graph). Mid-size problems are better off in the
+// graph, with its fused pass (8 problems of 108,900 determinants beside 16 of
+// 15,876: 358 ms all in the graph, 464 ms with the 8 on the single solver), so the
+// threshold lies between that size and 213,444, the next size measured.
+constexpr i64 kSingleND = 150000;
 
 void batch_solve(pyci_ctx *ctx, i64 nprob, const i64 *nbasis, const i64 *nup, const i64 *ndn, const double *ecore,
                  const double *const *one_mo, const double *const *two_mo, i64 nroot, double tol, int precision,

@@ -91,3 +91,24 @@ def test_no_root_skipped_with_singlet_start():
         np.testing.assert_allclose(np.sort(e), ref, rtol=0, atol=1e-7)
         two, _ = op.solve(n=2, tol=1e-10)
         np.testing.assert_allclose(np.sort(two), ref, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "f64"])
+def test_batch_routes_sizes_to_both_paths(precision):
+    """Problems of 150,000+ determinants take the single-problem solver beside the
+    replayed graph; mid-size ones (65,536..150,000) stay in the graph on its fused
+    pass; small ones on its batched kernels. Every root equals a per-problem FP64
+    solve of the same operator."""
+    shapes = [(11, 4, 4), (12, 4, 4), (9, 4, 4), (8, 3, 3), (11, 4, 4)]
+    probs = []
+    for i, (nb, nu, nd) in enumerate(shapes):
+        _, one, two = seeded_integrals(nb, 16, 900 + i)
+        probs.append((pyci.hamiltonian(0.1 * i, one, two), nu, nd))
+    ev, info = pyci.batch_solve(probs, n=2, tol=1e-8, precision=precision)
+    assert info["single_path"] == 1  # only the 245,025-determinant problem
+    assert all(info["converged"])
+    for (h, nu, nd), e in zip(probs, ev):
+        w = pyci.fullci_wfn(h.nbasis, nu, nd)
+        w.add_all_dets()
+        ref, _ = pyci.sparse_op(h, w).solve(n=2, tol=1e-10)
+        np.testing.assert_allclose(e, ref, rtol=0, atol=1e-8)
