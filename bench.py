@@ -297,7 +297,7 @@ def measured_traffic(fname):
 def device_copy(nbytes):
     """One device-to-device copy of nbytes through the HIP runtime (a plain copy
     kernel of known size, used to calibrate the HBM byte counters)."""
-    hip = ctypes.CDLL("libamdhip64.so")
+    hip = ctypes.CDLL("libamdhip64.so.7")  # by SONAME: the runtime the engine is bound to
     src, dst = ctypes.c_void_p(), ctypes.c_void_p()
     assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
     assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0

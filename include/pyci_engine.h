@@ -69,7 +69,11 @@ const char *pyci_last_error(void);
 int pyci_device_count(int *count);
 
 /* Context: binds a device and creates its stream. No counterpart in the
- * reference (its SparseOp runs on the host thread that calls it). */
+ * reference (its SparseOp runs on the host thread that calls it). The context
+ * also keeps the eigensolver's working memory and the matrix-free operator's
+ * workspace between calls (grown to the largest problem seen, released by
+ * pyci_ctx_destroy), and a second stream on which the solver runs the second
+ * parity sector. */
 int pyci_ctx_create(int device, pyci_ctx **ctx);
 void pyci_ctx_destroy(pyci_ctx *ctx);
 int pyci_ctx_synchronize(pyci_ctx *ctx);

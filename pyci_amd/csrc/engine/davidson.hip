@@ -608,23 +608,26 @@ struct Davidson {
     DevBuf<unsigned char> *ws;  // the matrix-free sigma's E workspace (nullptr: the context's)
     i64 nd;
     int k, mmax;
-    DevBuf<double> V, W, X, WX, T, rowbuf, rowout, partial, small;
-    DevBuf<double> RV, RW;   // thick-restart scratch (nd x 2k each)
-    DevBuf<double> dv;       // small device operands and results (layout: the o* offsets)
-    HostBuf<double> hb;      // page-locked mirror of dv for the per-iteration transfers
-    double *hd = nullptr;    // hb as the device sees it: the residual norms, norm pairs and
-                             // Rayleigh columns are written there by the kernels themselves
-    bool y_dev = false;      // the Ritz coefficients of the last Ritz pass are in dv
-    DevBuf<double> pb;       // per-block partials of the residual norms, the Gram-Schmidt
-                             // coefficients and the correction norms (layout: the p* offsets)
-    DevBuf<float> row32, out32;
+    // working memory: a set kept by the context (or owned, when none is given)
+    std::unique_ptr<DavBufs> owned;
+    DavBufs &bf;
+    DevBuf<double> &V, &W, &X, &WX, &T, &rowbuf, &rowout, &partial, &small;
+    DevBuf<double> &RV, &RW;  // thick-restart scratch (nd x 2k each)
+    DevBuf<double> &dv;       // small device operands and results (layout: the o* offsets)
+    HostBuf<double> &hb;      // page-locked mirror of dv for the per-iteration transfers
+    double *hd = nullptr;     // hb as the device sees it: the residual norms, norm pairs and
+                              // Rayleigh columns are written there by the kernels themselves
+    bool y_dev = false;       // the Ritz coefficients of the last Ritz pass are in dv
+    DevBuf<double> &pb;       // per-block partials of the residual norms, the Gram-Schmidt
+                              // coefficients and the correction norms (layout: the p* offsets)
+    DevBuf<float> &row32, &out32;
     std::vector<double> G;  // mmax x mmax row-major
     int m = 0;
     int nblk;
     int sector = 0;   // 0, or +1 / -1: the basis is kept in that parity sector of P
     i64 swap_nb = 0;  // beta-string count of P's index map
     bool exact = true;          // sigma outputs in the basis' sector to the last bit
-    DevBuf<double> dsym;        // in a sector: the diagonal made symmetric under P
+    DevBuf<double> &dsym;       // in a sector: the diagonal made symmetric under P
     const double *dg = nullptr; // the diagonal used by the preconditioner
     pyci_solve_stats st{};
     // dv / hb layout (doubles): Ritz coefficients Y (m x k) and theta go down; the
@@ -634,20 +637,26 @@ struct Davidson {
     size_t pRES, pN0, pN1, pN2, pC1, pC2;  // in pb: nblk entries per block-partial quantity
     static constexpr int kCgsMax = 128;    // basis size up to which Gram-Schmidt runs on partials
 
-    Davidson(pyci_op *o, int k_, int mmax_, hipStream_t stream = nullptr, DevBuf<unsigned char> *ws_ = nullptr)
-        : op(o), s(stream ? stream : o->ctx->stream), ws(ws_), nd(o->nrow), k(k_), mmax(mmax_) {
-        V.alloc((size_t)(nd * mmax));
-        W.alloc((size_t)(nd * mmax));
-        X.alloc((size_t)(nd * k));
-        WX.alloc((size_t)(nd * k));
-        T.alloc((size_t)(nd * k));
+    Davidson(pyci_op *o, int k_, int mmax_, hipStream_t stream = nullptr, DevBuf<unsigned char> *ws_ = nullptr,
+             DavBufs *kept = nullptr)
+        : op(o), s(stream ? stream : o->ctx->stream), ws(ws_), nd(o->nrow), k(k_), mmax(mmax_),
+          owned(kept ? nullptr : new DavBufs), bf(kept ? *kept : *owned), V(bf.V), W(bf.W), X(bf.X), WX(bf.WX),
+          T(bf.T), rowbuf(bf.rowbuf), rowout(bf.rowout), partial(bf.partial), small(bf.small), RV(bf.RV),
+          RW(bf.RW), dv(bf.dv), hb(bf.hb), pb(bf.pb), row32(bf.row32), out32(bf.out32), dsym(bf.dsym) {
+        // a kept set is only written by kernels of this run's stream; work of the
+        // previous run that used it was on the same stream
+        V.ensure((size_t)(nd * mmax));
+        W.ensure((size_t)(nd * mmax));
+        X.ensure((size_t)(nd * k));
+        WX.ensure((size_t)(nd * k));
+        T.ensure((size_t)(nd * k));
         // staging for sigma of up to max(k, start block) vectors at once
         const int kcols = std::max(k, std::min(2 * k, 16));
-        rowbuf.alloc((size_t)(nd * kcols));
-        rowout.alloc((size_t)(nd * kcols));
+        rowbuf.ensure((size_t)(nd * kcols));
+        rowout.ensure((size_t)(nd * kcols));
         nblk = (int)std::min<i64>(512, (nd + DT - 1) / DT);
-        partial.alloc((size_t)nblk * (size_t)std::max(mmax * std::max(k, kcols), 64) + 64);
-        small.alloc((size_t)(mmax * mmax + 4 * mmax + 64));
+        partial.ensure((size_t)nblk * (size_t)std::max(mmax * std::max(k, kcols), 64) + 64);
+        small.ensure((size_t)(mmax * mmax + 4 * mmax + 64));
         const size_t M = (size_t)mmax;
         oY = 0;
         oTH = oY + M * 16;
@@ -659,8 +668,8 @@ struct Davidson {
         oPAIR = oNRM + 3 * 16;
         oG = oPAIR + 2 * 16;
         osz = oG + M * 16;
-        dv.alloc(osz);
-        hb.alloc(osz);
+        dv.ensure(osz);
+        hb.ensure(osz);
         PE_HIP(hipHostGetDevicePointer((void **)&hd, hb.p, 0));
         const size_t NB = (size_t)nblk;
         pRES = 0;
@@ -669,7 +678,7 @@ struct Davidson {
         pN2 = pN1 + NB;
         pC1 = pN2 + NB;
         pC2 = pC1 + NB * kCgsMax;
-        pb.alloc(pC2 + NB * kCgsMax);
+        pb.ensure(pC2 + NB * kCgsMax);
         G.assign((size_t)mmax * mmax, 0.0);
         dg = op->diag.p;
     }
@@ -685,7 +694,7 @@ struct Davidson {
         exact = sgn == 0 || op->storage == PYCI_STORAGE_DIRECT;
         if (sgn == 0) return;
         PE_REQUIRE(nb > 0 && nb * nb == nd && nb < ((i64)1 << 31), PYCI_EINVAL, "parity sector needs nd = nb^2");
-        dsym.alloc((size_t)nd);
+        dsym.ensure((size_t)nd);
         k_symdiag<<<g1(), 256, 0, s>>>(op->diag.p, dsym.p, nd, nb);
         PE_LAUNCH_CHECK();
         dg = dsym.p;
@@ -761,10 +770,8 @@ struct Davidson {
         const double *src = V.p + (i64)c0 * nd;
         double *dst = W.p + (i64)c0 * nd;
         if (fp32) {
-            if (!row32.p || row32.n < (size_t)(nd * cnt)) {
-                row32.alloc((size_t)(nd * std::max(cnt, k)));
-                out32.alloc((size_t)(nd * std::max(cnt, k)));
-            }
+            row32.ensure((size_t)(nd * std::max(cnt, k)));
+            out32.ensure((size_t)(nd * std::max(cnt, k)));
             k_col2row<double, float><<<g1(), 256, 0, s>>>(nd, cnt, src, nd, row32.p);
             PE_LAUNCH_CHECK();
             sigma_f32(op, cnt, row32.p, out32.p, s, sector, ws);
@@ -1080,10 +1087,8 @@ struct Davidson {
     // becomes diag(w).
     void restart(int kk, const std::vector<double> &w, const std::vector<double> &y) {
         if (kk > k) {
-            if (!RV.p) {
-                RV.alloc((size_t)(nd * 2 * k));
-                RW.alloc((size_t)(nd * 2 * k));
-            }
+            RV.ensure((size_t)(nd * 2 * k));
+            RW.ensure((size_t)(nd * 2 * k));
             for (int j = 0; j < kk; ++j)
                 for (int i = 0; i < m; ++i) hb.p[oYK + (size_t)j * m + i] = y[(size_t)i * m + j];
             PE_HIP(hipMemcpyAsync(dv.p + oYK, hb.p + oYK, sizeof(double) * (size_t)m * kk, hipMemcpyHostToDevice, s));
@@ -1284,11 +1289,13 @@ struct Davidson {
 };
 
 // One Davidson run for the k lowest roots (of a parity sector of P when sector
-// != 0). Returns converged flag; theta (k) ascending and the Ritz vectors (k x nd,
-// host) of the run. stream / ws: where the run's kernels go and the matrix-free
-// sigma's workspace (nullptr: the context's); link: see Davidson::link.
+// != 0). Returns converged flag and theta (k) ascending; the Ritz vectors of the
+// run stay on the device in kept.X (nd x k). stream / ws: where the run's kernels
+// go and the matrix-free sigma's workspace (nullptr: the context's); kept: the
+// working memory set (the context keeps one per sector stream); link: see
+// Davidson::link.
 static bool solve_run(pyci_op *op, int k, const double *c0, i64 mm, i64 maxit, double tol, int precision, int sector,
-                      std::vector<double> &theta, std::vector<double> &hx, pyci_solve_stats &st,
+                      std::vector<double> &theta, DavBufs &kept, pyci_solve_stats &st,
                       const SectorLink *link = nullptr, hipStream_t stream = nullptr,
                       DevBuf<unsigned char> *ws = nullptr, bool *stopped_early = nullptr,
                       SectorLink *pub = nullptr) {
@@ -1296,12 +1303,13 @@ static bool solve_run(pyci_op *op, int k, const double *c0, i64 mm, i64 maxit, d
     hipStream_t s = stream ? stream : op->ctx->stream;
     const i64 nb = op->swap_nb;
     auto partner = [&](i64 J) { return (J % nb) * nb + J / nb; };
-    Davidson D(op, k, (int)mm, s, ws);
+    Davidson D(op, k, (int)mm, s, ws, &kept);
     D.set_sector(sector, nb);
     D.link = link;
     D.pub = pub;
-    std::vector<double> hd((size_t)nd);
-    PE_HIP(hipMemcpyAsync(hd.data(), op->diag.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
+    kept.hdiag.ensure((size_t)nd);
+    const double *hd = kept.hdiag.p;
+    PE_HIP(hipMemcpyAsync(kept.hdiag.p, op->diag.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
     // start block: unit vectors on the lowest diagonal entries (in a sector: the
     // sector's combination of a determinant and its partner), each with a small
@@ -1309,14 +1317,25 @@ static bool solve_run(pyci_op *op, int k, const double *c0, i64 mm, i64 maxit, d
     // 16, leaving room for k corrections)
     const int kb0 = (k == 1) ? 1 : (int)std::min<i64>(nd, std::max<i64>(k, std::min<i64>({(i64)2 * k, 16, mm - k})));
     const i64 want = std::min<i64>(nd, 4 * (i64)kb0 + 64);
-    std::vector<i64> idx((size_t)nd);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::partial_sort(idx.begin(), idx.begin() + want, idx.end(), [&](i64 a, i64 b) {
-        return hd[(size_t)a] < hd[(size_t)b] || (hd[(size_t)a] == hd[(size_t)b] && a < b);
-    });
+    // the `want` lowest diagonal entries in ascending order (ties by index), one pass
+    // with a bounded heap
+    auto lower = [&](i64 a, i64 b) { return hd[a] < hd[b] || (hd[a] == hd[b] && a < b); };
+    std::vector<i64> idx;
+    idx.reserve((size_t)want + 1);
+    for (i64 r = 0; r < nd; ++r) {
+        if ((i64)idx.size() < want) {
+            idx.push_back(r);
+            std::push_heap(idx.begin(), idx.end(), lower);
+        } else if (lower(r, idx.front())) {
+            std::pop_heap(idx.begin(), idx.end(), lower);
+            idx.back() = r;
+            std::push_heap(idx.begin(), idx.end(), lower);
+        }
+    }
+    std::sort_heap(idx.begin(), idx.end(), lower);
     std::vector<double> init;
-    std::vector<char> taken;
-    if (sector != 0) taken.assign((size_t)nd, 0);
+    std::vector<i64> taken;  // determinants already in a start vector of this sector
+    auto is_taken = [&](i64 d) { return std::find(taken.begin(), taken.end(), d) != taken.end(); };
     size_t cursor = 0;
     D.m = 0;
     // the caller's start vector (its part in this sector), unless that part vanishes
@@ -1354,8 +1373,9 @@ static bool solve_run(pyci_op *op, int k, const double *c0, i64 mm, i64 maxit, d
                     continue;
                 }
                 const i64 d1 = partner(d0);
-                if (taken[(size_t)d0] || (d1 == d0 && sector < 0)) continue;
-                taken[(size_t)d0] = taken[(size_t)d1] = 1;
+                if (is_taken(d0) || (d1 == d0 && sector < 0)) continue;
+                taken.push_back(d0);
+                taken.push_back(d1);
                 if (d1 == d0)
                     k_start<<<D.g1(), 256, 0, s>>>(col, nd, seed, amp, nb, (double)sector, d0, 1.0, -1, 0.0);
                 else
@@ -1404,8 +1424,6 @@ static bool solve_run(pyci_op *op, int k, const double *c0, i64 mm, i64 maxit, d
     }
     st = D.st;
     if (stopped_early) *stopped_early = D.stopped_early;
-    hx.resize((size_t)(nd * k));
-    PE_HIP(hipMemcpyAsync(hx.data(), D.X.p, sizeof(double) * nd * k, hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
     return ok;
 }
@@ -1434,7 +1452,12 @@ void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, 
     const i64 nb = op->swap_nb;
     const i64 odd_dim = (nb > 0) ? (nd - nb) / 2 : 0;
     const bool sectors = nb > 0 && odd_dim >= 2 * mm && nd - odd_dim >= 2 * mm;
-    std::vector<double> theta_a, hx_a, theta_b, hx_b;
+    std::vector<double> theta_a, theta_b;
+    // the context's working memory sets: [0] for the first (or only) run, [1] for
+    // the second sector's run
+    pyci_ctx *cx = op->ctx;
+    for (auto &d : cx->dav)
+        if (!d) d.reset(new DavBufs);
     pyci_solve_stats st_a{}, st_b{};
     std::vector<std::pair<double, std::pair<int, int>>> roots;  // (theta, (run, index))
     bool ok_a, ok_b = true;
@@ -1455,14 +1478,14 @@ void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, 
         std::thread tb([&] {
             try {
                 PE_HIP(hipSetDevice(ctx->device));
-                ok_b = solve_run(op, k, c0, mm, maxit, tol, precision, -1, theta_b, hx_b, st_b, &link,
+                ok_b = solve_run(op, k, c0, mm, maxit, tol, precision, -1, theta_b, *cx->dav[1], st_b, &link,
                                  ctx->stream2, &ctx->mf_ws2, &early);
             } catch (...) {
                 err_b = std::current_exception();
             }
         });
         try {
-            ok_a = solve_run(op, k, c0, mm, maxit, tol, precision, +1, theta_a, hx_a, st_a, nullptr, nullptr,
+            ok_a = solve_run(op, k, c0, mm, maxit, tol, precision, +1, theta_a, *cx->dav[0], st_a, nullptr, nullptr,
                              nullptr, nullptr, &link);
             link.theta = theta_a[(size_t)k - 1];
             link.iters = st_a.iterations;
@@ -1481,7 +1504,7 @@ void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, 
         std::stable_sort(roots.begin(), roots.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
         roots.resize((size_t)k);
     } else {
-        ok_a = solve_run(op, k, c0, mm, maxit, tol, precision, 0, theta_a, hx_a, st_a);
+        ok_a = solve_run(op, k, c0, mm, maxit, tol, precision, 0, theta_a, *cx->dav[0], st_a);
         for (int j = 0; j < k; ++j) roots.push_back({theta_a[(size_t)j], {0, j}});
     }
     if (stats) {
@@ -1496,12 +1519,14 @@ void davidson_solve(pyci_op *op, i64 n, const double *c0, i64 ncv, i64 maxiter, 
     }
     PE_REQUIRE(ok_a && ok_b, PYCI_ENOCONV, "did not converge");
     // reference order: descending for n > 1 (sparseop.cpp:134-144)
+    // the selected Ritz vectors straight from the runs' device buffers
     for (int j = 0; j < k; ++j) {
         const auto &r = roots[(size_t)((k > 1) ? (k - 1 - j) : 0)];
-        const std::vector<double> &hx = (r.second.first == 0) ? hx_a : hx_b;
         evals[j] = r.first + op->ecore;
-        std::memcpy(evecs + (i64)j * nd, hx.data() + (i64)r.second.second * nd, sizeof(double) * nd);
+        const double *x = cx->dav[r.second.first]->X.p + (i64)r.second.second * nd;
+        PE_HIP(hipMemcpyAsync(evecs + (i64)j * nd, x, sizeof(double) * nd, hipMemcpyDeviceToHost, cx->stream));
     }
+    PE_HIP(hipStreamSynchronize(cx->stream));
 }
 
 }  // namespace pe

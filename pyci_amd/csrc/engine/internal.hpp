@@ -67,6 +67,10 @@ struct DevBuf {
         return *this;
     }
     ~DevBuf() { release(); }
+    // at least count elements: keeps the allocation when it is large enough
+    void ensure(size_t count) {
+        if (!p || n < count) alloc(count);
+    }
     void alloc(size_t count) {
         release();
         if (count == 0) count = 1;
@@ -100,6 +104,9 @@ struct HostBuf {
     HostBuf(const HostBuf &) = delete;
     HostBuf &operator=(const HostBuf &) = delete;
     ~HostBuf() { release(); }
+    void ensure(size_t count) {
+        if (!p || n < count) alloc(count);
+    }
     void alloc(size_t count) {
         release();
         if (count == 0) count = 1;
@@ -121,6 +128,17 @@ struct HostBuf {
 
 }  // namespace pe
 
+namespace pe {
+// Working memory of one Davidson run (davidson.hip). The context keeps one set per
+// parity-sector stream between solves, so that a solve allocates nothing once the
+// sizes have been seen; the sets are released with the context.
+struct DavBufs {
+    DevBuf<double> V, W, X, WX, T, rowbuf, rowout, partial, small, RV, RW, dv, pb, dsym;
+    DevBuf<float> row32, out32;
+    HostBuf<double> hb, hdiag;
+};
+}  // namespace pe
+
 // ---------------------------------------------------------------------------
 // Handles
 // ---------------------------------------------------------------------------
@@ -134,6 +152,7 @@ struct pyci_ctx {
     // concurrently with the first (created on first use)
     hipStream_t stream2 = nullptr;
     pe::DevBuf<unsigned char> mf_ws2;
+    std::unique_ptr<pe::DavBufs> dav[2];  // solver working memory per sector stream
     double *noalloc_ws = nullptr;     // the fixed workspace of the allocation-free product on this device
 };
 

@@ -35,7 +35,9 @@ def _lib():
 
 
 def _hip():
-    h = ctypes.CDLL("libamdhip64.so")
+    # by its SONAME: the HIP runtime the engine is bound to (when torch was imported
+    # first, its bundled copy; "libamdhip64.so" would load a second runtime)
+    h = ctypes.CDLL("libamdhip64.so.7")
     h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
     h.hipFree.argtypes = [ctypes.c_void_p]
     h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
