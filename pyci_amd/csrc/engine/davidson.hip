@@ -1225,11 +1225,19 @@ struct Davidson {
                     improved = true;
                 }
             stall = improved ? 0 : stall + 1;
-            // stagnation: the FP32 phase hands over after 8 flat iterations; in FP64 a
-            // residual at the rounding floor is accepted after 25, and only 300 flat
-            // iterations count as failure (slow roots can plateau for dozens of
-            // iterations before they converge)
-            if (fp32 && stall >= 8) return leave(false);
+            // stagnation: the FP32 phase hands over after 8 flat iterations once every
+            // residual is within 1e-4 |theta| (near its rounding floor; a plateau far
+            // above it is the normal early course of the iteration: FullCI nbasis 12
+            // plateaued at |r| ~ 1 Eh, where handing over cost the FP32 phase 28 of its
+            // 44 iterations), or after 40 anywhere; in FP64 a residual at the rounding
+            // floor is accepted after 25, and only 300 flat iterations count as failure
+            // (slow roots can plateau for dozens of iterations before they converge)
+            if (fp32) {
+                bool near_floor = true;
+                for (int j = 0; j < k; ++j)
+                    if (!(res[j] <= 1e-4 * std::max(std::fabs(theta[j]), 1.0))) near_floor = false;
+                if ((stall >= 8 && near_floor) || stall >= 40) return leave(false);
+            }
             if (!fp32 && stall >= 25) {
                 bool tiny = true;
                 for (int j = 0; j < k; ++j)
