@@ -262,12 +262,13 @@ __global__ void __launch_bounds__(256) k_bctl(BView b) {
     double *res = b.res + (i64)p * BK_MAX, *best = b.best + (i64)p * BK_MAX;
     int *sel = b.sel + p * BK_MAX;
     int nun = 0;
-    bool floor = true, improved = false;
+    bool floor = true, improved = false, near_floor = true;
     for (int j = 0; j < k; ++j) {
         res[j] = rs[j];
         const double lim = b.tol * fmax(fabs(th[j]), eps23);
         if (!(rs[j] <= lim)) sel[nun++] = j;
         if (!(rs[j] <= fmax(lim, b.switch_tol * fmax(fabs(th[j]), 1.0)))) floor = false;
+        if (!(rs[j] <= 1e-4 * fmax(fabs(th[j]), 1.0))) near_floor = false;
         if (rs[j] < 0.9 * best[j]) {
             best[j] = rs[j];
             improved = true;
@@ -278,7 +279,9 @@ __global__ void __launch_bounds__(256) k_bctl(BView b) {
     st[S_REINIT] = 0;
     const bool fp32 = st[S_PHASE] == 1;
     bool done = (nun == 0);
-    bool switch_phase = fp32 && (done || floor || st[S_STALL] >= 8);
+    // as the single-problem solver: a stall hands the FP32 phase over only near its
+    // rounding floor (every residual within 1e-4 |theta|), or after 40 flat iterations
+    bool switch_phase = fp32 && (done || floor || (st[S_STALL] >= 8 && near_floor) || st[S_STALL] >= 40);
     if (!fp32 && !done && st[S_STALL] >= 25) {
         // as the single-problem solver: accept a residual at the rounding floor, fail
         // only after 300 flat iterations
