@@ -36,10 +36,30 @@ __device__ inline void tri_decode(int t, int &i, int &k) {
 
 #define GIX(p, q, r, s) ((((p) * n + (q)) * n + (r)) * n + (s))
 
-template <int KIND, bool FILL>
+// Sorted fill of the full FullCI space in colex order. There every single and
+// double excitation of a row is in the space and column = rank(alpha) NB +
+// rank(beta), so a row sorted by column is a run of segments, one per string a'
+// related to the row's alpha string a, in increasing rank of a': (a, b') for b' in
+// the sorted list of b, its singles and doubles (a' = a); (a', b') for b' in the
+// sorted list of b and its singles (a' a single of a); (a', b) alone (a' a double of
+// a). Every row has the same length, and an entry's position in its row is a segment
+// offset of the alpha string plus a rank within a list of the beta string, both
+// looked up by enumeration index: 0 the string itself, 1 + i nv + a the single
+// occ[i] -> vir[a], 1 + ns + po npv + pv the double occ[i], occ[k] -> vir[a], vir[l]
+// with (i < k) = pair po and (a < l) = pair pv (the decode order of k_rows). The
+// fill then writes each entry at its place and no sort is needed.
+struct SortedRows {
+    const int *seg;  // [NA][La]: offset in the row of the segment of alpha excitation e
+    const int *pf;   // [NB][LbF]: rank of beta excitation e among b, its singles and doubles
+    const int *ps;   // [NB][LbS]: rank of beta excitation e < LbS among b and its singles
+    int La, LbF, LbS;
+};
+
+template <int KIND, bool FILL, bool SORTED>
 __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i64 row1, i64 ncol, i64 *cnt_full,
                                               i64 *cnt_low, int *maxlen, const i64 *rowptr, int32_t *cols,
-                                              double *vals, double *diag) {
+                                              double *vals, double *diag, SortedRows T) {
+    static_assert(!SORTED || (FILL && KIND == PYCI_WFN_FULLCI), "sorted fill is for the full FullCI space");
     const int lane = threadIdx.x & 63;
     const i64 row = row0 + (i64)blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
     if (row >= row1) return;  // uniform per wavefront; no block barriers below
@@ -77,15 +97,20 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
     i64 out_base = 0;
     if (FILL) out_base = rowptr[row - row0];
     int written = 0, cfull = 0, clow = 0;
+    // sorted fill: the row's alpha and beta string ranks
+    const i64 ra = SORTED ? row / w.nsd : 0, rb = SORTED ? row - ra * w.nsd : 0;
     for (int e0 = 0; e0 < ncand; e0 += 64) {
         const int e = e0 + lane;
         const bool valid = e < ncand;
         int cls = 0, i = 0, a = 0, k = 0, l = 0;
+        int ea = 0, eb = 0, lb = 0;  // enumeration indices; beta list: 0 none, 1 singles, 2 all
         if (valid) {
             if (e < c0) {
                 cls = 0;
                 i = e / nvu;
                 a = e - i * nvu;
+                ea = 1 + e;
+                lb = 1;
             } else if (e < c1) {
                 cls = 1;
                 int t = e - c0, ia = t / nsd, kl = t - ia * nsd;
@@ -93,21 +118,29 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
                 a = ia - i * nvu;
                 k = kl / nvd;
                 l = kl - k * nvd;
+                ea = 1 + ia;
+                eb = 1 + kl;
+                lb = 1;
             } else if (e < c2) {
                 cls = 2;
                 int t = e - c1, po = t / npv_u, pv = t - po * npv_u;
                 tri_decode(po, i, k);
                 tri_decode(pv, a, l);
+                ea = 1 + nsu + t;
             } else if (e < c3) {
                 cls = 3;
                 int t = e - c2;
                 i = t / nvd;
                 a = t - i * nvd;
+                eb = 1 + t;
+                lb = 2;
             } else {
                 cls = 4;
                 int t = e - c3, po = t / npv_d, pv = t - po * npv_d;
                 tri_decode(po, i, k);
                 tri_decode(pv, a, l);
+                eb = 1 + nsd + t;
+                lb = 2;
             }
         }
         // gather orbital numbers from the owning lanes (uniform control flow)
@@ -181,6 +214,15 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
                 }
             }
         }
+        if constexpr (SORTED) {
+            if (col >= 0) {
+                const int pos = T.seg[ra * T.La + ea] +
+                                (lb == 1 ? T.ps[rb * T.LbS + eb] : lb == 2 ? T.pf[rb * T.LbF + eb] : 0);
+                cols[out_base + pos] = (int32_t)col;
+                vals[out_base + pos] = val;
+            }
+            continue;
+        }
         const u64 hit = __ballot(col >= 0);
         if (FILL) {
             if (col >= 0) {
@@ -238,8 +280,9 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
             }
             diag[row - row0] = d;
             if (has_diag) {
-                cols[out_base + written] = (int32_t)row;
-                vals[out_base + written] = d;
+                const int pos = SORTED ? T.seg[ra * T.La] + T.pf[rb * T.LbF] : written;
+                cols[out_base + pos] = (int32_t)row;
+                vals[out_base + pos] = d;
             }
         } else {
             if (has_diag) {
@@ -362,14 +405,152 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_rows_global(i64 nrow, con
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tables of the sorted fill (see SortedRows): one block per string r of nocc
+// electrons in n orbitals. Its La = 1 + ns + nd related strings (itself, ns = nocc nv
+// singles, nd = C(nocc, 2) C(nv, 2) doubles) are ranked in colex order (bitonic sort
+// in LDS); seg (alpha side) receives each one's segment offset in a row, with segment
+// sizes LbF (the string itself), LbS (a single) and 1 (a double); pf / ps (beta side)
+// its rank among all La / among the string and its singles.
+// ---------------------------------------------------------------------------
+constexpr int TAB_CAP = 4096;
+
+__global__ void __launch_bounds__(256) k_string_tables(int n, int nocc, const i64 *binom, int LbF, int LbS, int *seg,
+                                                       int *pf, int *ps) {
+    __shared__ int key[TAB_CAP];
+    __shared__ unsigned short pay[TAB_CAP];
+    __shared__ int part[2][256];
+    const i64 r = blockIdx.x;
+    const int tid = threadIdx.x, nv = n - nocc, ns = nocc * nv, npv = nv * (nv - 1) / 2;
+    const int L = 1 + ns + (nocc * (nocc - 1) / 2) * npv;
+    // string of colex rank r
+    u64 str = 0;
+    {
+        i64 rank = r;
+        int p = n;
+        for (int k = nocc; k >= 1; --k) {
+            int q = p - 1;
+            while (binom[q * 65 + k] > rank) --q;
+            str |= 1ULL << q;
+            rank -= binom[q * 65 + k];
+            p = q;
+        }
+    }
+    const u64 vir = ~str & full_mask(n);
+    int N = 1;
+    while (N < L) N <<= 1;
+    for (int e = tid; e < N; e += 256) {
+        if (e < L) {
+            u64 t = str;
+            if (e >= 1 && e <= ns) {
+                const int i = (e - 1) / nv, a = (e - 1) - i * nv;
+                t ^= (1ULL << nth_bit(str, i)) | (1ULL << nth_bit(vir, a));
+            } else if (e > ns) {
+                const int d = e - 1 - ns, po = d / npv, pv = d - po * npv;
+                int i, k, a, l;
+                tri_decode(po, i, k);
+                tri_decode(pv, a, l);
+                t ^= (1ULL << nth_bit(str, i)) | (1ULL << nth_bit(str, k)) | (1ULL << nth_bit(vir, a)) |
+                     (1ULL << nth_bit(vir, l));
+            }
+            key[e] = (int)colex_rank(t, binom);
+            pay[e] = (unsigned short)e;
+        } else {
+            key[e] = 0x7fffffff;
+            pay[e] = 0xffff;
+        }
+    }
+    __syncthreads();
+    for (int size = 2; size <= N; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = tid; t < N / 2; t += 256) {
+                int p, q;
+                if (stride == (size >> 1)) {  // first step of a merge: compare mirrored pairs
+                    const int blk = t / stride, o = t - blk * stride;
+                    p = blk * size + o;
+                    q = blk * size + size - 1 - o;
+                } else {
+                    const int blk = t / stride, o = t - blk * stride;
+                    p = blk * 2 * stride + o;
+                    q = p + stride;
+                }
+                if (key[q] < key[p]) {
+                    const int tk = key[p];
+                    key[p] = key[q];
+                    key[q] = tk;
+                    const unsigned short tp = pay[p];
+                    pay[p] = pay[q];
+                    pay[q] = tp;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // segment offsets and ranks among the string and its singles: block scan over
+    // contiguous chunks of the sorted order
+    const int CH = (L + 255) / 256, j0 = min(L, tid * CH), j1 = min(L, j0 + CH);
+    int sz = 0, c1 = 0;
+    for (int j = j0; j < j1; ++j) {
+        const int e = pay[j];
+        sz += e == 0 ? LbF : e <= ns ? LbS : 1;
+        c1 += e <= ns ? 1 : 0;
+    }
+    part[0][tid] = sz;
+    part[1][tid] = c1;
+    __syncthreads();
+    if (tid == 0) {
+        int a = 0, b = 0;
+        for (int t = 0; t < 256; ++t) {
+            const int x = part[0][t], y = part[1][t];
+            part[0][t] = a;
+            part[1][t] = b;
+            a += x;
+            b += y;
+        }
+    }
+    __syncthreads();
+    int o = part[0][tid], q = part[1][tid];
+    for (int j = j0; j < j1; ++j) {
+        const int e = pay[j];
+        if (seg) seg[r * L + e] = o;
+        if (pf) pf[r * L + e] = j;
+        if (ps && e <= ns) ps[r * (1 + ns) + e] = q;
+        o += e == 0 ? LbF : e <= ns ? LbS : 1;
+        q += e <= ns ? 1 : 0;
+    }
+}
+
+// Row pointers (already scanned) and per-row lower counts (entries with column <=
+// row: the diagonal's position + 1) of rows row0 .. row0 + nr - 1 of the sorted fill.
+__global__ void k_sorted_meta(i64 row0, i64 nr, i64 rowlen, i64 nsd, SortedRows T, i64 *rowptr, i64 *lowcnt) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i <= nr; i += (i64)gridDim.x * blockDim.x) {
+        rowptr[i] = i * rowlen;
+        if (i < nr) {
+            const i64 row = row0 + i, ra = row / nsd, rb = row - ra * nsd;
+            lowcnt[i] = (i64)T.seg[ra * T.La] + (i64)T.pf[rb * T.LbF] + 1;
+        } else {
+            lowcnt[i] = 0;
+        }
+    }
+}
+
 template <int KIND>
 static void launch_rows(bool fill, const WfnView &w, const HamView &H, i64 row0, i64 row1, i64 ncol, i64 *cf,
-                        i64 *cl, int *mx, const i64 *rp, int32_t *cols, double *vals, double *diag, hipStream_t s) {
+                        i64 *cl, int *mx, const i64 *rp, int32_t *cols, double *vals, double *diag, hipStream_t s,
+                        const SortedRows *T = nullptr) {
     unsigned g = grid_for(row1 - row0, ROWS_PER_BLOCK);
+    if constexpr (KIND == PYCI_WFN_FULLCI) {
+        if (T) {
+            k_rows<KIND, true, true><<<g, 256, 0, s>>>(w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, *T);
+            PE_LAUNCH_CHECK();
+            return;
+        }
+    }
+    const SortedRows none{nullptr, nullptr, nullptr, 0, 0, 0};
     if (fill)
-        k_rows<KIND, true><<<g, 256, 0, s>>>(w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag);
+        k_rows<KIND, true, false><<<g, 256, 0, s>>>(w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, none);
     else
-        k_rows<KIND, false><<<g, 256, 0, s>>>(w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag);
+        k_rows<KIND, false, false><<<g, 256, 0, s>>>(w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, none);
     PE_LAUNCH_CHECK();
 }
 
@@ -385,6 +566,53 @@ static void dispatch_rows(int kind, bool fill, const WfnView &w, const HamView &
         launch_rows<PYCI_WFN_GENCI>(fill, w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, s);
 }
 
+// Sorted fill of rows row0 .. row1 - 1 of the full FullCI space (see SortedRows):
+// tables, row pointers, lower counts, one fill pass. False where the space or the
+// column range is not the full one, or a string has more related strings than the
+// table kernel holds.
+static bool build_rows_sorted(pyci_op *op, const pyci_wfn *wfn, const WfnView &w, const HamView &H, i64 row0,
+                              i64 row1, i64 ncol, DevBuf<i64> &rowptr_seg, DevBuf<int32_t> &cols,
+                              DevBuf<double> &vals, DevBuf<i64> &lowcnt_seg, DevBuf<double> &diag_seg, i64 &nnz_seg,
+                              i64 &low_total) {
+    if (op->kind != PYCI_WFN_FULLCI || !wfn->colex || ncol != wfn->ndet) return false;
+    const i64 n = wfn->nbasis, nu = wfn->nup, nd = wfn->ndn, vu = n - nu, vd = n - nd;
+    const i64 La = 1 + nu * vu + (nu * (nu - 1) / 2) * (vu * (vu - 1) / 2);
+    const i64 LbF = 1 + nd * vd + (nd * (nd - 1) / 2) * (vd * (vd - 1) / 2), LbS = 1 + nd * vd;
+    if (La > TAB_CAP || LbF > TAB_CAP) return false;
+    const i64 NA = binom_host(n, nu), NB = binom_host(n, nd);
+    const i64 rowlen = LbF + nu * vu * LbS + (La - 1 - nu * vu);
+    const i64 nr = row1 - row0;
+    PE_REQUIRE(rowlen * nr < (i64)1 << 40, PYCI_ENOMEM, "sparse operator too large");
+    pyci_ctx *ctx = op->ctx;
+    hipStream_t s = ctx->stream;
+    DevBuf<int> seg((size_t)(NA * La)), pf((size_t)(NB * LbF)), ps((size_t)(NB * LbS));
+    k_string_tables<<<(unsigned)NA, 256, 0, s>>>((int)n, (int)nu, w.binom, (int)LbF, (int)LbS, seg.p, nullptr,
+                                                 nullptr);
+    PE_LAUNCH_CHECK();
+    k_string_tables<<<(unsigned)NB, 256, 0, s>>>((int)n, (int)nd, w.binom, (int)LbF, (int)LbS, nullptr, pf.p, ps.p);
+    PE_LAUNCH_CHECK();
+    const SortedRows T{seg.p, pf.p, ps.p, (int)La, (int)LbF, (int)LbS};
+    rowptr_seg.alloc((size_t)(nr + 1));
+    lowcnt_seg.alloc((size_t)(nr + 1));
+    diag_seg.alloc((size_t)(nr > 0 ? nr : 1));
+    k_sorted_meta<<<std::min<unsigned>(4096u, grid_for(nr + 1, 256)), 256, 0, s>>>(row0, nr, rowlen, w.nsd, T,
+                                                                                  rowptr_seg.p, lowcnt_seg.p);
+    PE_LAUNCH_CHECK();
+    {
+        DevBuf<i64> tmp((size_t)(nr + 1));
+        PE_HIP(hipMemcpyAsync(tmp.p, lowcnt_seg.p, sizeof(i64) * (nr + 1), hipMemcpyDeviceToDevice, s));
+        low_total = exclusive_scan(ctx, tmp.p, nr + 1);
+    }
+    nnz_seg = rowlen * nr;
+    cols.alloc((size_t)(nnz_seg > 0 ? nnz_seg : 1));
+    vals.alloc((size_t)(nnz_seg > 0 ? nnz_seg : 1));
+    if (nr > 0)
+        launch_rows<PYCI_WFN_FULLCI>(true, w, H, row0, row1, ncol, nullptr, nullptr, nullptr, rowptr_seg.p, cols.p,
+                                     vals.p, diag_seg.p, s, &T);
+    PE_HIP(hipStreamSynchronize(s));  // the tables are freed on return
+    return true;
+}
+
 void build_rows(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn, i64 row0, i64 row1, i64 ncol,
                 DevBuf<i64> &rowptr_seg, DevBuf<int32_t> &cols, DevBuf<double> &vals, DevBuf<i64> &lowcnt_seg,
                 DevBuf<double> &diag_seg, i64 &nnz_seg, i64 &low_total) {
@@ -393,6 +621,9 @@ void build_rows(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn, i64 row0,
     const i64 nr = row1 - row0;
     WfnView w = view_of(wfn);
     HamView H = view_of(ham);
+    if (build_rows_sorted(op, wfn, w, H, row0, row1, ncol, rowptr_seg, cols, vals, lowcnt_seg, diag_seg, nnz_seg,
+                          low_total))
+        return;
     rowptr_seg.alloc((size_t)(nr + 1));
     lowcnt_seg.alloc((size_t)(nr + 1));
     diag_seg.alloc((size_t)(nr > 0 ? nr : 1));
