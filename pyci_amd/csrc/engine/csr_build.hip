@@ -19,6 +19,7 @@
 namespace pe {
 
 constexpr int ROWS_PER_BLOCK = 4;  // 4 wavefronts x 64 lanes
+constexpr int GENCI_MAXPV = 64 * 63 / 2;  // virtual orbital pairs of a 64-bit GenCI determinant
 
 __device__ inline int nth_bit(u64 x, int t) {
     for (int s = 0; s < t && x; ++s) x &= x - 1;
@@ -60,6 +61,8 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
                                               i64 *cnt_low, int *maxlen, const i64 *rowptr, int32_t *cols,
                                               double *vals, double *diag, SortedRows T) {
     static_assert(!SORTED || (FILL && KIND == PYCI_WFN_FULLCI), "sorted fill is for the full FullCI space");
+    // GenCI: per wave, the virtual orbital pairs (a < l) of its row, packed a | l << 8
+    __shared__ unsigned short genci_vp[KIND == PYCI_WFN_GENCI ? ROWS_PER_BLOCK * GENCI_MAXPV : 1];
     const int lane = threadIdx.x & 63;
     const i64 row = row0 + (i64)blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
     if (row >= row1) return;  // uniform per wavefront; no block barriers below
@@ -99,69 +102,14 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
     int written = 0, cfull = 0, clow = 0;
     // sorted fill: the row's alpha and beta string ranks
     const i64 ra = SORTED ? row / w.nsd : 0, rb = SORTED ? row - ra * w.nsd : 0;
-    for (int e0 = 0; e0 < ncand; e0 += 64) {
-        const int e = e0 + lane;
-        const bool valid = e < ncand;
-        int cls = 0, i = 0, a = 0, k = 0, l = 0;
-        int ea = 0, eb = 0, lb = 0;  // enumeration indices; beta list: 0 none, 1 singles, 2 all
-        if (valid) {
-            if (e < c0) {
-                cls = 0;
-                i = e / nvu;
-                a = e - i * nvu;
-                ea = 1 + e;
-                lb = 1;
-            } else if (e < c1) {
-                cls = 1;
-                int t = e - c0, ia = t / nsd, kl = t - ia * nsd;
-                i = ia / nvu;
-                a = ia - i * nvu;
-                k = kl / nvd;
-                l = kl - k * nvd;
-                ea = 1 + ia;
-                eb = 1 + kl;
-                lb = 1;
-            } else if (e < c2) {
-                cls = 2;
-                int t = e - c1, po = t / npv_u, pv = t - po * npv_u;
-                tri_decode(po, i, k);
-                tri_decode(pv, a, l);
-                ea = 1 + nsu + t;
-            } else if (e < c3) {
-                cls = 3;
-                int t = e - c2;
-                i = t / nvd;
-                a = t - i * nvd;
-                eb = 1 + t;
-                lb = 2;
-            } else {
-                cls = 4;
-                int t = e - c3, po = t / npv_d, pv = t - po * npv_d;
-                tri_decode(po, i, k);
-                tri_decode(pv, a, l);
-                eb = 1 + nsd + t;
-                lb = 2;
-            }
-        }
-        // gather orbital numbers from the owning lanes (uniform control flow)
-        const int pi = __shfl(packed, i), pa = __shfl(packed, a), pk = __shfl(packed, k), pl = __shfl(packed, l);
+    // One candidate per lane (called by the whole wave): class cls, orbitals ii -> jj
+    // (and kk -> ll for the second excitation; for the 1-1 class ii -> jj is alpha,
+    // kk -> ll beta), enumeration indices ea / eb in beta list lb for the sorted fill.
+    // Forms the determinant, looks it up, evaluates the element and counts or stores it.
+    auto visit = [&](bool valid, int cls, int ii, int jj, int kk, int ll, int ea, int eb, int lb) {
         i64 col = -1;
         double val = 0.0;
         if (valid) {
-            const bool alpha = (cls <= 2);
-            const int ii = alpha ? (pi & 255) : ((pi >> 16) & 255);
-            const int jj = alpha ? ((pa >> 8) & 255) : ((pa >> 24) & 255);
-            int kk, ll;
-            if (cls == 1) {
-                kk = (pk >> 16) & 255;
-                ll = (pl >> 24) & 255;
-            } else if (alpha) {
-                kk = pk & 255;
-                ll = (pl >> 8) & 255;
-            } else {
-                kk = (pk >> 16) & 255;
-                ll = (pl >> 24) & 255;
-            }
             u64 nu = up, nd = dn;
             if (KIND == PYCI_WFN_DOCI) {
                 nu = up ^ (1ULL << ii) ^ (1ULL << jj);
@@ -221,7 +169,7 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
                 cols[out_base + pos] = (int32_t)col;
                 vals[out_base + pos] = val;
             }
-            continue;
+            return;
         }
         const u64 hit = __ballot(col >= 0);
         if (FILL) {
@@ -234,6 +182,100 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
         } else {
             cfull += __popcll(hit);
             clow += __popcll(__ballot(col >= 0 && col < row));
+        }
+    };
+    if constexpr (KIND == PYCI_WFN_GENCI) {
+        // GenCI: the singles flat, then the doubles with the occupied pair uniform
+        // over the wave and the virtual pairs from a per-wave LDS table built once per
+        // row (no per-candidate index decode). The order of the candidates is free:
+        // the row is sorted by column afterwards.
+        for (int e0 = 0; e0 < nsu; e0 += 64) {
+            const int e = e0 + lane, ec = e < nsu ? e : 0, i = ec / nvu, a = ec - i * nvu;
+            const int ii = __shfl(packed, i) & 255, jj = (__shfl(packed, a) >> 8) & 255;
+            visit(e < nsu, 0, ii, jj, 0, 0, 0, 0, 0);
+        }
+        unsigned short *vp = genci_vp + (threadIdx.x >> 6) * GENCI_MAXPV;
+        for (int t0 = 0; t0 < npv_u; t0 += 64) {
+            const int t = t0 + lane;
+            int a, l;
+            tri_decode(t < npv_u ? t : 0, a, l);
+            const int va = (__shfl(packed, a) >> 8) & 255, vl = (__shfl(packed, l) >> 8) & 255;
+            if (t < npv_u) vp[t] = (unsigned short)(va | (vl << 8));
+        }
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+        for (int po = 0; po < npo_u; ++po) {
+            int i, k;
+            tri_decode(po, i, k);
+            const int ii = __shfl(packed, i) & 255, kk = __shfl(packed, k) & 255;
+            for (int t0 = 0; t0 < npv_u; t0 += 64) {
+                const int t = t0 + lane;
+                const unsigned pv = vp[t < npv_u ? t : 0];
+                visit(t < npv_u, 2, ii, (int)(pv & 255), kk, (int)(pv >> 8), 0, 0, 0);
+            }
+        }
+    } else {
+        for (int e0 = 0; e0 < ncand; e0 += 64) {
+            const int e = e0 + lane;
+            const bool valid = e < ncand;
+            int cls = 0, i = 0, a = 0, k = 0, l = 0;
+            int ea = 0, eb = 0, lb = 0;  // enumeration indices; beta list: 0 none, 1 singles, 2 all
+            if (valid) {
+                if (e < c0) {
+                    cls = 0;
+                    i = e / nvu;
+                    a = e - i * nvu;
+                    ea = 1 + e;
+                    lb = 1;
+                } else if (e < c1) {
+                    cls = 1;
+                    int t = e - c0, ia = t / nsd, kl = t - ia * nsd;
+                    i = ia / nvu;
+                    a = ia - i * nvu;
+                    k = kl / nvd;
+                    l = kl - k * nvd;
+                    ea = 1 + ia;
+                    eb = 1 + kl;
+                    lb = 1;
+                } else if (e < c2) {
+                    cls = 2;
+                    int t = e - c1, po = t / npv_u, pv = t - po * npv_u;
+                    tri_decode(po, i, k);
+                    tri_decode(pv, a, l);
+                    ea = 1 + nsu + t;
+                } else if (e < c3) {
+                    cls = 3;
+                    int t = e - c2;
+                    i = t / nvd;
+                    a = t - i * nvd;
+                    eb = 1 + t;
+                    lb = 2;
+                } else {
+                    cls = 4;
+                    int t = e - c3, po = t / npv_d, pv = t - po * npv_d;
+                    tri_decode(po, i, k);
+                    tri_decode(pv, a, l);
+                    eb = 1 + nsd + t;
+                    lb = 2;
+                }
+            }
+            // gather orbital numbers from the owning lanes (uniform control flow)
+            const int pi = __shfl(packed, i), pa = __shfl(packed, a), pk = __shfl(packed, k), pl = __shfl(packed, l);
+            const bool alpha = (cls <= 2);
+            const int ii = alpha ? (pi & 255) : ((pi >> 16) & 255);
+            const int jj = alpha ? ((pa >> 8) & 255) : ((pa >> 24) & 255);
+            int kk, ll;
+            if (cls == 1) {
+                kk = (pk >> 16) & 255;
+                ll = (pl >> 24) & 255;
+            } else if (alpha) {
+                kk = pk & 255;
+                ll = (pl >> 8) & 255;
+            } else {
+                kk = (pk >> 16) & 255;
+                ll = (pl >> 24) & 255;
+            }
+            visit(valid, cls, ii, jj, kk, ll, ea, eb, lb);
         }
     }
     if (lane == 0) {

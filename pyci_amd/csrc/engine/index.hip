@@ -32,6 +32,7 @@ WfnView view_of(const pyci_wfn *w) {
     v.binom = w->ctx->binom.p;
     v.hkeys = w->hkeys.p;
     v.hvals = w->hvals.p;
+    v.htags = w->htags.p;
     v.hmask = w->hcap ? (u64)(w->hcap - 1) : 0;
     return v;
 }
@@ -63,25 +64,29 @@ __global__ void k_colex_check(WfnView w, int *bad) {
     if (!ok) atomicOr(bad, 1);
 }
 
-__global__ void k_hash_clear(i64 *vals, i64 n) {
+__global__ void k_hash_clear(i64 *vals, u64 *tags, i64 n) {
     i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) vals[i] = -1;
+    if (i < n / 8) tags[i] = 0;
 }
 
 // Inserts dets[first..ndet) with value = position. Dets are distinct, so a slot
 // is claimed by CAS on its value and the key is written by the claimer; no
 // lookups run concurrently with this kernel.
-__global__ void k_hash_insert(const u64 *dets, int nkey, i64 first, i64 ndet, u64 *keys, i64 *vals, u64 mask) {
+__global__ void k_hash_insert(const u64 *dets, int nkey, i64 first, i64 ndet, u64 *keys, i64 *vals, u64 *tags,
+                              u64 mask) {
     i64 i = first + (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ndet) return;
     u64 a = dets[nkey * i], b = (nkey == 2) ? dets[nkey * i + 1] : 0ULL;
-    u64 h = hash_key(a, b) & mask;
+    const u64 hk = hash_key(a, b);
+    u64 h = hk & mask;
     for (;;) {
         unsigned long long old = atomicCAS((unsigned long long *)&vals[h], (unsigned long long)(-1LL),
                                            (unsigned long long)i);
         if (old == (unsigned long long)(-1LL)) {
             keys[2 * h] = a;
             keys[2 * h + 1] = b;
+            reinterpret_cast<unsigned char *>(tags)[h] = (unsigned char)((hk >> 57) | 0x80ULL);
             return;
         }
         h = (h + 1) & mask;
@@ -122,6 +127,7 @@ void wfn_build_index(pyci_wfn *w) {
     if (w->colex) {
         w->hkeys.release();
         w->hvals.release();
+        w->htags.release();
         w->hcap = 0;
         return;
     }
@@ -131,11 +137,12 @@ void wfn_build_index(pyci_wfn *w) {
     w->hcap = cap;
     w->hkeys.alloc((size_t)(2 * cap));
     w->hvals.alloc((size_t)cap);
-    k_hash_clear<<<grid_for(cap, 256), 256, 0, s>>>(w->hvals.p, cap);
+    w->htags.alloc((size_t)(cap / 8));
+    k_hash_clear<<<grid_for(cap, 256), 256, 0, s>>>(w->hvals.p, w->htags.p, cap);
     PE_LAUNCH_CHECK();
     if (w->ndet > 0) {
         k_hash_insert<<<grid_for(w->ndet, 256), 256, 0, s>>>(w->dets.p, w->nkey, 0, w->ndet, w->hkeys.p, w->hvals.p,
-                                                              (u64)(cap - 1));
+                                                              w->htags.p, (u64)(cap - 1));
         PE_LAUNCH_CHECK();
     }
 }
@@ -151,7 +158,7 @@ void wfn_index_append(pyci_wfn *w, i64 first) {
     i64 n = w->ndet - first;
     if (n > 0) {
         k_hash_insert<<<grid_for(n, 256), 256, 0, s>>>(w->dets.p, w->nkey, first, w->ndet, w->hkeys.p, w->hvals.p,
-                                                       (u64)(w->hcap - 1));
+                                                       w->htags.p, (u64)(w->hcap - 1));
         PE_LAUNCH_CHECK();
     }
 }

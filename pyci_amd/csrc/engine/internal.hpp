@@ -174,8 +174,10 @@ struct pyci_wfn {
     bool colex = false;
     pe::i64 nsd = 1;  // beta (or only) string count, for FullCI colex rank
     // otherwise: open-addressing hash (key words per slot = 2, value = index, -1 empty)
+    // with one tag byte per slot (0 empty, else 0x80 | the hash's top 7 bits), 8 per word
     pe::DevBuf<pe::u64> hkeys;
     pe::DevBuf<pe::i64> hvals;
+    pe::DevBuf<pe::u64> htags;
     pe::i64 hcap = 0;
 };
 
@@ -250,6 +252,7 @@ struct WfnView {
     const i64 *binom;  // 65 x 65
     const u64 *hkeys;
     const i64 *hvals;
+    const u64 *htags;
     u64 hmask;
 };
 
@@ -273,8 +276,12 @@ __host__ __device__ inline u64 mix64(u64 x) {
     return x;
 }
 
+// Slot hash of determinant (a, b) in the device index: one finaliser over a and an
+// odd multiple of b (b = 0 for one-spin kinds, where the multiply folds away: the
+// GenCI row builder probes ~42,000 candidates per row, and the hash is a large part
+// of each probe's instructions).
 __host__ __device__ inline u64 hash_key(u64 a, u64 b) {
-    return mix64(mix64(a + 0x9e3779b97f4a7c15ULL) ^ (b + 0x632be59bd9b4e019ULL));
+    return mix64((a + 0x9e3779b97f4a7c15ULL) ^ (b * 0xd6e8feb86659fd93ULL));
 }
 
 __device__ inline i64 colex_rank(u64 det, const i64 *binom) {
@@ -295,12 +302,31 @@ __device__ inline i64 wfn_lookup(const WfnView &w, u64 a, u64 b) {
         if (w.nkey == 2) return colex_rank(a, w.binom) * w.nsd + colex_rank(b, w.binom);
         return colex_rank(a, w.binom);
     }
-    u64 h = hash_key(a, b) & w.hmask;
+    // Linear probing over the tag bytes, eight slots per load: a probe run ends at the
+    // first empty byte, and only slots whose tag matches are compared by key, so a
+    // determinant that is not in the set usually costs one 8-byte load. Byte tests are
+    // word-wide: for x with bytes 0 or >= 0x80, (x - 0x0101..) & ~x & 0x8080.. flags
+    // exactly the zero bytes; applied to x ^ tag it flags the matching bytes (and at
+    // most spurious bytes above a true match, which the key compare rejects).
+    const u64 hk = hash_key(a, b), ones = 0x0101010101010101ULL, highs = 0x8080808080808080ULL;
+    const u64 tagw = ((hk >> 57) | 0x80ULL) * ones;
+    u64 h = hk & w.hmask;
     for (;;) {
-        i64 v = w.hvals[h];
-        if (v < 0) return -1;
-        if (w.hkeys[2 * h] == a && w.hkeys[2 * h + 1] == b) return v;
-        h = (h + 1) & w.hmask;
+        const u64 x = w.htags[h >> 3];
+        const int j0 = (int)(h & 7);
+        const u64 keep = ~0ULL << (8 * j0);  // slots h .. end of the word
+        const u64 empty = ((x - ones) & ~x & highs) & keep;
+        const u64 y = x ^ tagw;
+        u64 match = ((y - ones) & ~y & highs) & keep;
+        // matches before the first empty slot, in probe order
+        if (empty) match &= (empty & (~empty + 1)) - 1;
+        while (match) {
+            const u64 s = (h & ~7ULL) + (u64)(__builtin_ctzll(match) >> 3);
+            if (w.hkeys[2 * s] == a && w.hkeys[2 * s + 1] == b) return w.hvals[s];
+            match &= match - 1;
+        }
+        if (empty) return -1;
+        h = ((h | 7ULL) + 1ULL) & w.hmask;
     }
 }
 
