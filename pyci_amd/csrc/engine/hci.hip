@@ -12,8 +12,8 @@
 // then the reference's loop order inside a row), which is deterministic:
 //   pass 1 counts passing new candidates (with repeats) to size a hash set,
 //   pass 2 inserts each candidate's 64-bit key with the minimum position,
-//   pass 3 counts per row the candidates that own their key's minimum,
-//   pass 4 writes owners in row order after a scan.
+//   then the claimed slots, one per new determinant, are sorted by that
+//   position (a radix sort) and written in that order.
 // One wavefront handles one source row; the outer (i, j) loops run uniformly
 // and lanes cover the inner excitation block, so ballot compaction preserves
 // the reference order.
@@ -22,6 +22,8 @@
 // insertion threshold is then a looser eps_pool, and each candidate records its
 // heat-bath score max_i |H_ji c_i| and whether plain add_hci(eps) takes it.
 #include <algorithm>
+
+#include <hipcub/hipcub.hpp>
 
 #include "internal.hpp"
 
@@ -66,36 +68,13 @@ __device__ inline u64 set_insert(const NewSet &S, u64 key, u64 p) {
     return ~0ULL;
 }
 
-__device__ inline u64 set_find_slot(const NewSet &S, u64 key) {
-    u64 h = mix64(key) & S.mask;
-    for (u64 step = 0; step <= S.mask; ++step) {
-        u64 k = S.keys[h];
-        if (k == key) return h;
-        if (k == EMPTY_KEY) return ~0ULL;
-        h = (h + 1) & S.mask;
-    }
-    return ~0ULL;
-}
-
-__device__ inline u64 set_find_pos(const NewSet &S, u64 key) {
-    u64 h = mix64(key) & S.mask;
-    for (u64 step = 0; step <= S.mask; ++step) {
-        u64 k = S.keys[h];
-        if (k == key) return S.pos[h];
-        if (k == EMPTY_KEY) return ~0ULL;
-        h = (h + 1) & S.mask;
-    }
-    return ~0ULL;
-}
-
 #define GIX(p, q, r, s) ((((p) * n + (q)) * n + (r)) * n + (s))
 
-// PASS 1: count, 2: insert, 3: owners per row, 4: write owners,
-// 5: ENPT2 numerators (insert, then add the signed H_ai c_i of every passing candidate).
+// PASS 1: count, 2: insert, 5: ENPT2 numerators (insert, then add the signed
+// H_ai c_i of every passing candidate).
 template <int KIND, int PASS>
 __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double *coeffs, double eps, double eps_plain,
-                                             i64 rows_stride, NewSet S, unsigned long long *count, i64 *row_cnt,
-                                             const i64 *row_off, u64 *out) {
+                                             i64 rows_stride, NewSet S, unsigned long long *count) {
     const int lane = threadIdx.x & 63;
     const i64 row = (i64)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= w.ndet) return;
@@ -116,8 +95,6 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
     const int packed = nth(up, lane) | (nth(~up & m, lane) << 8) | (nth(dn, lane) << 16) | (nth(~dn & m, lane) << 24);
     (void)tmp;
     u64 local = 0;                 // PASS 1 count
-    int owned = 0;                 // PASS 3/4 running count in this row
-    i64 obase = (PASS == 4) ? row_off[row] : 0;
     i64 e_base = 0;                // running position inside the row (reference order)
 
     // Evaluates one candidate; returns true when it passes and is new.
@@ -151,15 +128,6 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
                     if (fabs(val) > eps_plain_i) atomicOr(&S.plain[slot], 1);
                 }
             }
-        } else {
-            const bool own = pass && (set_find_pos(S, key) == pos);
-            const u64 b = __ballot(own);
-            if (PASS == 4 && own) {
-                const i64 o = obase + owned + __popcll(b & ((1ULL << lane) - 1ULL));
-                out[(i64)w.nkey * o] = nu;
-                if (w.nkey == 2) out[2 * o + 1] = nd;
-            }
-            owned += __popcll(b);
         }
     };
 
@@ -279,10 +247,7 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
             }
         }
     }
-    if (lane == 0) {
-        if (PASS == 1) atomicAdd(count, (unsigned long long)local);
-        if (PASS == 3) row_cnt[row] = owned;
-    }
+    if (lane == 0 && PASS == 1) atomicAdd(count, (unsigned long long)local);
 }
 
 __global__ void k_set_clear(u64 *keys, u64 *pos, u64 *score, int *plain, i64 n) {
@@ -298,30 +263,79 @@ __global__ void k_set_clear(u64 *keys, u64 *pos, u64 *score, int *plain, i64 n) 
 
 template <int PASS>
 static void launch_hci(int kind, const WfnView &w, const HamView &H, const double *c, double eps, double eps_plain,
-                       i64 stride, const NewSet &S, unsigned long long *count, i64 *rc, const i64 *ro, u64 *out,
-                       hipStream_t s) {
+                       i64 stride, const NewSet &S, unsigned long long *count, hipStream_t s) {
     unsigned g = grid_for(w.ndet, 4);
     if (kind == PYCI_WFN_DOCI)
-        k_hci<PYCI_WFN_DOCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count, rc, ro, out);
+        k_hci<PYCI_WFN_DOCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count);
     else if (kind == PYCI_WFN_FULLCI)
-        k_hci<PYCI_WFN_FULLCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count, rc, ro, out);
+        k_hci<PYCI_WFN_FULLCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count);
     else
-        k_hci<PYCI_WFN_GENCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count, rc, ro, out);
+        k_hci<PYCI_WFN_GENCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count);
     PE_LAUNCH_CHECK();
 }
 
 void wfn_index_append(pyci_wfn *w, i64 first);
 
-// Passes 1-3 of the selection: the set of new candidates (|H_ji| > eps/|c_i|,
-// not in the wave function) with their first-occurrence positions, and the
-// exclusive-scanned per-row owner counts in rc (nnew = total).
+// Claimed slots of the set as (first position, slot) pairs, in no particular order:
+// one cursor increment per wave.
+__global__ void k_set_owners(NewSet S, i64 cap, unsigned long long *cursor, u64 *pos_out, u64 *slot_out) {
+    const int lane = threadIdx.x & 63;
+    const i64 step = (i64)gridDim.x * blockDim.x;
+    for (i64 h0 = (i64)blockIdx.x * blockDim.x + (threadIdx.x & ~63); h0 < cap; h0 += step) {
+        const i64 h = h0 + lane;
+        const bool claimed = h < cap && S.keys[h] != EMPTY_KEY;
+        const u64 b = __ballot(claimed);
+        unsigned long long base = 0;
+        if (lane == 0 && b) base = atomicAdd(cursor, (unsigned long long)__popcll(b));
+        base = __shfl(base, 0);
+        if (claimed) {
+            const unsigned long long o = base + (unsigned long long)__popcll(b & ((1ULL << lane) - 1ULL));
+            pos_out[o] = S.pos[h];
+            slot_out[o] = (u64)h;
+        }
+    }
+}
+
+// String of colex rank r with k electrons in nb orbitals.
+__device__ inline u64 colex_unrank(i64 r, int nb, int k, const i64 *binom) {
+    u64 str = 0;
+    int p = nb;
+    for (int q = k; q >= 1; --q) {
+        int t = p - 1;
+        while (binom[t * 65 + q] > r) --t;
+        str |= 1ULL << t;
+        r -= binom[t * 65 + q];
+        p = t;
+    }
+    return str;
+}
+
+// The determinants of the sorted owner slots: a one-word key is the determinant,
+// a FullCI key is rank(alpha) * nsd + rank(beta) (set_key).
+__global__ void k_emit_owners(WfnView w, NewSet S, const u64 *slots, i64 n, u64 *out) {
+    const i64 o = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    const u64 key = S.keys[slots[o]];
+    if (w.nkey == 1) {
+        out[o] = key;
+    } else {
+        const i64 ra = (i64)(key / (u64)w.nsd), rb = (i64)(key - (u64)ra * (u64)w.nsd);
+        out[2 * o] = colex_unrank(ra, w.nb, w.nup, w.binom);
+        out[2 * o + 1] = colex_unrank(rb, w.nb, w.ndn, w.binom);
+    }
+}
+
+// Passes 1-2 of the selection: the set of new candidates (|H_ji| > eps/|c_i|,
+// not in the wave function) with their first-occurrence positions; then its
+// claimed slots ordered by that position (a radix sort), which is the order of
+// first occurrence in the serial enumeration: slots[0 .. nnew).
 struct HciRun {
-    DevBuf<u64> keys, pos, score;
+    DevBuf<u64> keys, pos, score, slots;
     DevBuf<int> plain, ovf;
     DevBuf<unsigned long long> cnt;
-    DevBuf<i64> rc;
     NewSet S{};
     i64 stride = 0, nnew = 0;
+    unsigned long long S_unique_host = 0;  // claimed slots after pass 2
 };
 
 static void hci_collect(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double eps, double eps_plain, bool pool,
@@ -345,7 +359,7 @@ static void hci_collect(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double e
     R.ovf.alloc(1);
     PE_HIP(hipMemsetAsync(R.cnt.p, 0, sizeof(unsigned long long) * 2, s));
     R.S = NewSet{nullptr, nullptr, 0, R.cnt.p + 1, R.ovf.p};
-    launch_hci<1>(wfn->kind, w, H, dc, eps, eps_plain, R.stride, R.S, R.cnt.p, nullptr, nullptr, nullptr, s);
+    launch_hci<1>(wfn->kind, w, H, dc, eps, eps_plain, R.stride, R.S, R.cnt.p, s);
     unsigned long long upper = 0;
     PE_HIP(hipMemcpyAsync(&upper, R.cnt.p, sizeof(upper), hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
@@ -372,20 +386,41 @@ static void hci_collect(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double e
             R.S.score = R.score.p;
             R.S.plain = R.plain.p;
         }
-        launch_hci<2>(wfn->kind, w, H, dc, eps, eps_plain, R.stride, R.S, nullptr, nullptr, nullptr, nullptr, s);
+        launch_hci<2>(wfn->kind, w, H, dc, eps, eps_plain, R.stride, R.S, nullptr, s);
         unsigned long long uniq = 0;
         int of = 0;
         PE_HIP(hipMemcpyAsync(&uniq, R.cnt.p + 1, sizeof(uniq), hipMemcpyDeviceToHost, s));
         PE_HIP(hipMemcpyAsync(&of, R.ovf.p, sizeof(int), hipMemcpyDeviceToHost, s));
         PE_HIP(hipStreamSynchronize(s));
-        if (!of && 2 * (i64)uniq <= cap) break;
+        if (!of && 2 * (i64)uniq <= cap) {
+            R.S_unique_host = uniq;
+            break;
+        }
         PE_REQUIRE(cap < ((i64)1 << 34), PYCI_ENOMEM, "selection hash set too large");
         cap <<= 2;
     }
-    R.rc.alloc((size_t)(nd + 1));
-    PE_HIP(hipMemsetAsync(R.rc.p, 0, sizeof(i64) * (nd + 1), s));
-    launch_hci<3>(wfn->kind, w, H, dc, eps, eps_plain, R.stride, R.S, nullptr, R.rc.p, nullptr, nullptr, s);
-    R.nnew = exclusive_scan(ctx, R.rc.p, nd + 1);
+    // every claimed slot is one new determinant with its first position
+    const i64 nnew = (i64)R.S_unique_host;
+    R.nnew = nnew;
+    if (nnew == 0) return;
+    PE_REQUIRE(nnew < ((i64)1 << 31), PYCI_ENOMEM, "too many new determinants in one selection step");
+    DevBuf<u64> pos_in((size_t)nnew), slot_in((size_t)nnew), pos_out((size_t)nnew);
+    R.slots.alloc((size_t)nnew);
+    PE_HIP(hipMemsetAsync(R.cnt.p, 0, sizeof(unsigned long long), s));
+    k_set_owners<<<(unsigned)std::min<i64>(4096, grid_for(cap, 256)), 256, 0, s>>>(R.S, cap, R.cnt.p, pos_in.p,
+                                                                                   slot_in.p);
+    PE_LAUNCH_CHECK();
+    // positions are below nd * stride
+    int end_bit = 1;
+    while (end_bit < 64 && (long double)((unsigned long long)1 << end_bit) < (long double)nd * (long double)R.stride)
+        ++end_bit;
+    size_t tmp_bytes = 0;
+    PE_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, pos_in.p, pos_out.p, slot_in.p, R.slots.p,
+                                              (int)nnew, 0, end_bit, s));
+    DevBuf<unsigned char> tmp(tmp_bytes);
+    PE_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, pos_in.p, pos_out.p, slot_in.p, R.slots.p,
+                                              (int)nnew, 0, end_bit, s));
+    PE_HIP(hipStreamSynchronize(s));  // the sort's buffers leave scope
 }
 
 i64 add_hci_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, double eps) {
@@ -407,8 +442,9 @@ i64 add_hci_device(pyci_ham *ham, pyci_wfn *wfn, const double *coeffs_host, doub
         PE_HIP(hipStreamSynchronize(s));
         wfn->dets = std::move(nb);
     }
-    launch_hci<4>(wfn->kind, view_of(wfn), view_of(ham), dc.p, eps, eps, R.stride, R.S, nullptr, nullptr, R.rc.p,
-                  wfn->dets.p + nd * wfn->nkey, s);
+    k_emit_owners<<<grid_for(nnew, 256), 256, 0, s>>>(view_of(wfn), R.S, R.slots.p, nnew,
+                                                      wfn->dets.p + nd * wfn->nkey);
+    PE_LAUNCH_CHECK();
     PE_HIP(hipStreamSynchronize(s));
     wfn->ndet = nd + nnew;
     wfn_index_append(wfn, nd);
@@ -422,15 +458,13 @@ __global__ void k_int_to_i64(const int *src, i64 n, i64 *dst) {
     if (i <= n) dst[i] = (i < n) ? (i64)src[i] : 0;
 }
 
-// Score and plain flag of each pooled candidate, looked up by its set key.
-__global__ void k_pool_scores(WfnView w, NewSet S, const u64 *cand, i64 n, double *hb, int *plain) {
+// Score and plain flag of each pooled candidate, from its slot.
+__global__ void k_pool_scores(NewSet S, const u64 *slots, i64 n, double *hb, int *plain) {
     const i64 o = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= n) return;
-    const u64 a = cand[(i64)w.nkey * o];
-    const u64 b = (w.nkey == 2) ? cand[2 * o + 1] : 0ULL;
-    const u64 slot = set_find_slot(S, set_key(w, a, b));
-    hb[o] = (slot == ~0ULL) ? 0.0 : __longlong_as_double((long long)S.score[slot]);
-    plain[o] = (slot == ~0ULL) ? 0 : S.plain[slot];
+    const u64 slot = slots[o];
+    hb[o] = __longlong_as_double((long long)S.score[slot]);
+    plain[o] = S.plain[slot];
 }
 
 void hci_pool_device(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double eps, double eps_pool, HciPool &P) {
@@ -446,8 +480,9 @@ void hci_pool_device(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double eps,
     P.hb.alloc((size_t)R.nnew);
     P.plain.alloc((size_t)R.nnew);
     const WfnView w = view_of(wfn);
-    launch_hci<4>(wfn->kind, w, view_of(ham), dc, eps_pool, eps, R.stride, R.S, nullptr, nullptr, R.rc.p, P.dets.p, s);
-    k_pool_scores<<<grid_for(P.n, 256), 256, 0, s>>>(w, R.S, P.dets.p, P.n, P.hb.p, P.plain.p);
+    k_emit_owners<<<grid_for(P.n, 256), 256, 0, s>>>(w, R.S, R.slots.p, P.n, P.dets.p);
+    PE_LAUNCH_CHECK();
+    k_pool_scores<<<grid_for(P.n, 256), 256, 0, s>>>(R.S, R.slots.p, P.n, P.hb.p, P.plain.p);
     PE_LAUNCH_CHECK();
     // number of candidates plain add_hci(eps) appends
     DevBuf<i64> cnt((size_t)(P.n + 1));
@@ -567,7 +602,7 @@ double enpt2_device(pyci_ham *ham, pyci_wfn *wfn_in, const double *coeffs_host, 
     PE_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * 2, s));
     NewSet S{nullptr, nullptr, 0, cnt.p + 1, ovf.p};
     // pass 1: passing external candidates (with repeats) bound the set size
-    launch_hci<1>(wfn->kind, w, H, dc.p, eps, eps, 0, S, cnt.p, nullptr, nullptr, nullptr, s);
+    launch_hci<1>(wfn->kind, w, H, dc.p, eps, eps, 0, S, cnt.p, s);
     unsigned long long upper = 0;
     PE_HIP(hipMemcpyAsync(&upper, cnt.p, sizeof(upper), hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
@@ -592,7 +627,7 @@ double enpt2_device(pyci_ham *ham, pyci_wfn *wfn_in, const double *coeffs_host, 
         S = NewSet{keys.p, pos.p, (u64)(cap - 1), cnt.p + 1, ovf.p};
         S.num = num.p;
         S.kdet = kdet.p;
-        launch_hci<5>(wfn->kind, w, H, dc.p, eps, eps, 0, S, nullptr, nullptr, nullptr, nullptr, s);
+        launch_hci<5>(wfn->kind, w, H, dc.p, eps, eps, 0, S, nullptr, s);
         int of = 0;
         PE_HIP(hipMemcpyAsync(&uniq, cnt.p + 1, sizeof(uniq), hipMemcpyDeviceToHost, s));
         PE_HIP(hipMemcpyAsync(&of, ovf.p, sizeof(int), hipMemcpyDeviceToHost, s));
