@@ -26,15 +26,6 @@ __device__ inline int nth_bit(u64 x, int t) {
     return x ? __builtin_ctzll(x) : 0;
 }
 
-// t = k(k-1)/2 + i with i < k
-__device__ inline void tri_decode(int t, int &i, int &k) {
-    int kk = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)t)) * 0.5f);
-    while (kk * (kk - 1) / 2 > t) --kk;
-    while ((kk + 1) * kk / 2 <= t) ++kk;
-    k = kk;
-    i = t - kk * (kk - 1) / 2;
-}
-
 #define GIX(p, q, r, s) ((((p) * n + (q)) * n + (r)) * n + (s))
 
 // Sorted fill of the full FullCI space in colex order. There every single and

@@ -70,11 +70,91 @@ __device__ inline u64 set_insert(const NewSet &S, u64 key, u64 p) {
 
 #define GIX(p, q, r, s) ((((p) * n + (q)) * n + (r)) * n + (s))
 
+// Heat-bath lists of a GenCI Hamiltonian (n <= 64 spin-orbitals). For every
+// orbital pair p < q (pair index P = q (q - 1) / 2 + p) the pairs r < s ordered by
+// |V| descending, V = two_mo[p][q][r][s] - two_mo[p][q][s][r] (the double-excitation
+// element of add_hci, hci.cpp:219-220, with the same operands, so the same bits),
+// stored with its sign; for every (i, a) an upper bound of the single-excitation
+// element: |h_ia| + sum over all q of |two_mo[i][q][a][q] - two_mo[i][q][q][a]|.
+// A row then visits, per occupied pair, only the list prefix above eps / |c_i|, and
+// computes a single's element only where the bound reaches it.
+struct HbLists {
+    const double *v = nullptr;         // [P][npair]
+    const unsigned short *rs = nullptr;  // [P][npair]: r | s << 8
+    const double *sb = nullptr;        // [i][a]
+    int npair = 0;
+};
+constexpr int HB_CAP = 2048;  // n (n - 1) / 2 <= 2016 for n <= 64
+
+__global__ void __launch_bounds__(256) k_hb_lists(HamView H, int npair, double *lv, unsigned short *lrs) {
+    __shared__ double key[HB_CAP];
+    __shared__ unsigned short idx[HB_CAP];
+    const int n = H.n, tid = threadIdx.x;
+    const double *two = H.two;
+    int p, q;
+    tri_decode((int)blockIdx.x, p, q);
+    int N = 1;
+    while (N < npair) N <<= 1;
+    for (int j = tid; j < N; j += 256) {
+        if (j < npair) {
+            int r, s;
+            tri_decode(j, r, s);
+            key[j] = fabs(two[GIX(p, q, r, s)] - two[GIX(p, q, s, r)]);
+        } else {
+            key[j] = -1.0;
+        }
+        idx[j] = (unsigned short)j;
+    }
+    __syncthreads();
+    // bitonic sort, descending by |V|
+    for (int size = 2; size <= N; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = tid; t < N / 2; t += 256) {
+                const int blk = t / stride, o = t - blk * stride;
+                int a, b;
+                if (stride == (size >> 1)) {
+                    a = blk * size + o;
+                    b = blk * size + size - 1 - o;
+                } else {
+                    a = blk * 2 * stride + o;
+                    b = a + stride;
+                }
+                if (key[a] < key[b]) {
+                    const double tk = key[a];
+                    key[a] = key[b];
+                    key[b] = tk;
+                    const unsigned short ti = idx[a];
+                    idx[a] = idx[b];
+                    idx[b] = ti;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const i64 base = (i64)blockIdx.x * npair;
+    for (int t = tid; t < npair; t += 256) {
+        int r, s;
+        tri_decode(idx[t], r, s);
+        lv[base + t] = two[GIX(p, q, r, s)] - two[GIX(p, q, s, r)];
+        lrs[base + t] = (unsigned short)(r | (s << 8));
+    }
+}
+
+__global__ void k_hb_single_bound(HamView H, double *sb) {
+    const int n = H.n, e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= n * n) return;
+    const double *one = H.one, *two = H.two;
+    const int i = e / n, a = e - i * n;
+    double b = fabs(one[i * n + a]);
+    for (int q = 0; q < n; ++q) b += fabs(two[GIX(i, q, a, q)] - two[GIX(i, q, q, a)]);
+    sb[e] = b;
+}
+
 // PASS 1: count, 2: insert, 5: ENPT2 numerators (insert, then add the signed
 // H_ai c_i of every passing candidate).
 template <int KIND, int PASS>
 __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double *coeffs, double eps, double eps_plain,
-                                             i64 rows_stride, NewSet S, unsigned long long *count) {
+                                             i64 rows_stride, NewSet S, unsigned long long *count, HbLists L) {
     const int lane = threadIdx.x & 63;
     const i64 row = (i64)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= w.ndet) return;
@@ -131,7 +211,61 @@ __global__ void __launch_bounds__(256) k_hci(WfnView w, HamView H, const double 
         }
     };
 
-    if (KIND == PYCI_WFN_DOCI) {
+    if constexpr (KIND == PYCI_WFN_GENCI) {
+        // Position of a candidate in the reference's row order (for i, for j: single
+        // (i, j), then doubles (k > i, l > j)): the single (i, j) at off(i, j), the
+        // double (i, j, k, l) at off(i, j) + 1 + (k - i - 1)(nv - 1 - j) + (l - j - 1).
+        const int no = nou, nv = nvu;
+        const i64 hv = (i64)nv * (nv - 1) / 2;
+        auto off = [&](int i, int j) -> i64 {
+            return (i64)i * nv + hv * ((i64)i * (no - 1) - (i64)i * (i - 1) / 2) + j +
+                   (i64)(no - 1 - i) * ((i64)j * (nv - 1) - (i64)j * (j - 1) / 2);
+        };
+        const u64 rbase = (u64)row * (u64)rows_stride;
+        // singles, where the bound reaches the threshold
+        for (int e0 = 0; e0 < no * nv; e0 += 64) {
+            const int e = e0 + lane;
+            const bool in = e < no * nv;
+            const int ec = in ? e : 0, i = ec / nv, j = ec - i * nv;
+            const int ii = __shfl(packed, i) & 255, jj = (__shfl(packed, j) >> 8) & 255;
+            const bool live = in && L.sb[ii * n + jj] * (1.0 + 1e-12) > eps_i;
+            double val = 0.0;
+            int sg = 1;
+            if (live) {
+                double v = one[ii * n + jj];
+                for (u64 x = up; x; x &= x - 1) {
+                    int q = __builtin_ctzll(x);
+                    v += two[GIX(ii, q, jj, q)] - two[GIX(ii, q, q, jj)];
+                }
+                val = v;
+                if (PASS == 5) sg = phase1(up, ii, jj);
+            }
+            consider(live, up ^ (1ULL << ii) ^ (1ULL << jj), 0ULL, val, rbase + (u64)off(i, j), sg);
+        }
+        // doubles: per occupied pair, the list prefix above the threshold
+        for (int po = 0; po < no * (no - 1) / 2; ++po) {
+            int i, k;
+            tri_decode(po, i, k);
+            const int ii = __shfl(packed, i) & 255, kk = __shfl(packed, k) & 255;
+            const i64 lb = (i64)(kk * (kk - 1) / 2 + ii) * L.npair;
+            for (int t0 = 0; t0 < L.npair; t0 += 64) {
+                const int t = t0 + lane;
+                const bool in = t < L.npair;
+                const double V = in ? L.v[lb + t] : 0.0;
+                const bool above = in && fabs(V) > eps_i;
+                if (__ballot(above) == 0) break;  // the list is sorted: nothing further passes
+                const unsigned rs = in ? L.rs[lb + t] : 0u;
+                const int r = (int)(rs & 255u), s2 = (int)(rs >> 8);
+                const bool free = above && !((up >> r) & 1ULL) && !((up >> s2) & 1ULL);
+                // r and s as indices among the virtual orbitals
+                const int j = r - __popcll(up & ((1ULL << r) - 1ULL)), l = s2 - __popcll(up & ((1ULL << s2) - 1ULL));
+                int sg = 1;
+                if (PASS == 5 && free) sg = phase2(up, ii, kk, r, s2);
+                const u64 pos = rbase + (u64)(off(i, j) + 1 + (i64)(k - i - 1) * (nv - 1 - j) + (l - j - 1));
+                consider(free, up ^ (1ULL << ii) ^ (1ULL << kk) ^ (1ULL << r) ^ (1ULL << s2), 0ULL, V, pos, sg);
+            }
+        }
+    } else if (KIND == PYCI_WFN_DOCI) {
         const int total = nou * nvu;
         for (int e0 = 0; e0 < total; e0 += 64) {
             const int e = e0 + lane;
@@ -261,16 +395,37 @@ __global__ void k_set_clear(u64 *keys, u64 *pos, u64 *score, int *plain, i64 n) 
     }
 }
 
+// The heat-bath lists of a GenCI selection (HbLists), built for one call.
+struct HbTables {
+    DevBuf<double> v, sb;
+    DevBuf<unsigned short> rs;
+    HbLists L;
+    void build(const HamView &H, hipStream_t s) {
+        const int n = H.n, npair = n * (n - 1) / 2;
+        PE_REQUIRE(npair <= HB_CAP, PYCI_EUNSUPPORTED, "too many orbitals for the heat-bath lists");
+        v.alloc((size_t)std::max(1, npair * npair));
+        rs.alloc((size_t)std::max(1, npair * npair));
+        sb.alloc((size_t)std::max(1, n * n));
+        if (npair > 0) {
+            k_hb_lists<<<(unsigned)npair, 256, 0, s>>>(H, npair, v.p, rs.p);
+            PE_LAUNCH_CHECK();
+        }
+        k_hb_single_bound<<<grid_for((i64)n * n, 256), 256, 0, s>>>(H, sb.p);
+        PE_LAUNCH_CHECK();
+        L = HbLists{v.p, rs.p, sb.p, npair};
+    }
+};
+
 template <int PASS>
 static void launch_hci(int kind, const WfnView &w, const HamView &H, const double *c, double eps, double eps_plain,
-                       i64 stride, const NewSet &S, unsigned long long *count, hipStream_t s) {
+                       i64 stride, const NewSet &S, unsigned long long *count, hipStream_t s, const HbLists &L) {
     unsigned g = grid_for(w.ndet, 4);
     if (kind == PYCI_WFN_DOCI)
-        k_hci<PYCI_WFN_DOCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count);
+        k_hci<PYCI_WFN_DOCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count, L);
     else if (kind == PYCI_WFN_FULLCI)
-        k_hci<PYCI_WFN_FULLCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count);
+        k_hci<PYCI_WFN_FULLCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count, L);
     else
-        k_hci<PYCI_WFN_GENCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count);
+        k_hci<PYCI_WFN_GENCI, PASS><<<g, 256, 0, s>>>(w, H, c, eps, eps_plain, stride, S, count, L);
     PE_LAUNCH_CHECK();
 }
 
@@ -359,7 +514,9 @@ static void hci_collect(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double e
     R.ovf.alloc(1);
     PE_HIP(hipMemsetAsync(R.cnt.p, 0, sizeof(unsigned long long) * 2, s));
     R.S = NewSet{nullptr, nullptr, 0, R.cnt.p + 1, R.ovf.p};
-    launch_hci<1>(wfn->kind, w, H, dc, eps, eps_plain, R.stride, R.S, R.cnt.p, s);
+    HbTables hb;
+    if (wfn->kind == PYCI_WFN_GENCI) hb.build(H, s);
+    launch_hci<1>(wfn->kind, w, H, dc, eps, eps_plain, R.stride, R.S, R.cnt.p, s, hb.L);
     unsigned long long upper = 0;
     PE_HIP(hipMemcpyAsync(&upper, R.cnt.p, sizeof(upper), hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
@@ -386,7 +543,7 @@ static void hci_collect(pyci_ham *ham, pyci_wfn *wfn, const double *dc, double e
             R.S.score = R.score.p;
             R.S.plain = R.plain.p;
         }
-        launch_hci<2>(wfn->kind, w, H, dc, eps, eps_plain, R.stride, R.S, nullptr, s);
+        launch_hci<2>(wfn->kind, w, H, dc, eps, eps_plain, R.stride, R.S, nullptr, s, hb.L);
         unsigned long long uniq = 0;
         int of = 0;
         PE_HIP(hipMemcpyAsync(&uniq, R.cnt.p + 1, sizeof(uniq), hipMemcpyDeviceToHost, s));
@@ -602,7 +759,9 @@ double enpt2_device(pyci_ham *ham, pyci_wfn *wfn_in, const double *coeffs_host, 
     PE_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * 2, s));
     NewSet S{nullptr, nullptr, 0, cnt.p + 1, ovf.p};
     // pass 1: passing external candidates (with repeats) bound the set size
-    launch_hci<1>(wfn->kind, w, H, dc.p, eps, eps, 0, S, cnt.p, s);
+    HbTables hb;
+    if (wfn->kind == PYCI_WFN_GENCI) hb.build(H, s);
+    launch_hci<1>(wfn->kind, w, H, dc.p, eps, eps, 0, S, cnt.p, s, hb.L);
     unsigned long long upper = 0;
     PE_HIP(hipMemcpyAsync(&upper, cnt.p, sizeof(upper), hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
@@ -627,7 +786,7 @@ double enpt2_device(pyci_ham *ham, pyci_wfn *wfn_in, const double *coeffs_host, 
         S = NewSet{keys.p, pos.p, (u64)(cap - 1), cnt.p + 1, ovf.p};
         S.num = num.p;
         S.kdet = kdet.p;
-        launch_hci<5>(wfn->kind, w, H, dc.p, eps, eps, 0, S, nullptr, s);
+        launch_hci<5>(wfn->kind, w, H, dc.p, eps, eps, 0, S, nullptr, s, hb.L);
         int of = 0;
         PE_HIP(hipMemcpyAsync(&uniq, cnt.p + 1, sizeof(uniq), hipMemcpyDeviceToHost, s));
         PE_HIP(hipMemcpyAsync(&of, ovf.p, sizeof(int), hipMemcpyDeviceToHost, s));

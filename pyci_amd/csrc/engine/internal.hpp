@@ -351,6 +351,15 @@ __device__ inline int phase2(u64 det, int i1, int i2, int a1, int a2) {
 
 __device__ inline u64 full_mask(int nb) { return nb >= 64 ? ~0ULL : ((1ULL << nb) - 1ULL); }
 
+// Pair index t = k (k - 1) / 2 + i with i < k, decoded.
+__device__ inline void tri_decode(int t, int &i, int &k) {
+    int kk = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)t)) * 0.5f);
+    while (kk * (kk - 1) / 2 > t) --kk;
+    while ((kk + 1) * kk / 2 <= t) ++kk;
+    k = kk;
+    i = t - kk * (kk - 1) / 2;
+}
+
 // ---------------------------------------------------------------------------
 // Host helpers
 // ---------------------------------------------------------------------------
