@@ -47,10 +47,14 @@ struct SortedRows {
     int La, LbF, LbS;
 };
 
+// Hit bitmap (bm, nchunk words per row, or null): the count pass stores each
+// chunk's ballot of candidates found in the set, and the fill pass then looks up
+// only those lanes and skips chunks without any (the candidates are enumerated in
+// the same chunks by both passes).
 template <int KIND, bool FILL, bool SORTED>
 __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i64 row1, i64 ncol, i64 *cnt_full,
                                               i64 *cnt_low, int *maxlen, const i64 *rowptr, int32_t *cols,
-                                              double *vals, double *diag, SortedRows T) {
+                                              double *vals, double *diag, SortedRows T, u64 *bm, i64 nchunk) {
     static_assert(!SORTED || (FILL && KIND == PYCI_WFN_FULLCI), "sorted fill is for the full FullCI space");
     // GenCI: per wave, the virtual orbital pairs (a < l) of its row, packed a | l << 8
     __shared__ unsigned short genci_vp[KIND == PYCI_WFN_GENCI ? ROWS_PER_BLOCK * GENCI_MAXPV : 1];
@@ -97,9 +101,13 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
     // (and kk -> ll for the second excitation; for the 1-1 class ii -> jj is alpha,
     // kk -> ll beta), enumeration indices ea / eb in beta list lb for the sorted fill.
     // Forms the determinant, looks it up, evaluates the element and counts or stores it.
+    u64 *bmrow = bm ? bm + (row - row0) * nchunk : nullptr;
+    i64 chunk = 0;
     auto visit = [&](bool valid, int cls, int ii, int jj, int kk, int ll, int ea, int eb, int lb) {
         i64 col = -1;
         double val = 0.0;
+        if (FILL && bmrow) valid = valid && ((bmrow[chunk] >> lane) & 1ULL);
+        ++chunk;
         if (valid) {
             u64 nu = up, nd = dn;
             if (KIND == PYCI_WFN_DOCI) {
@@ -171,6 +179,7 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
             }
             written += __popcll(hit);
         } else {
+            if (bmrow && lane == 0) bmrow[chunk - 1] = hit;
             cfull += __popcll(hit);
             clow += __popcll(__ballot(col >= 0 && col < row));
         }
@@ -570,33 +579,49 @@ __global__ void k_sorted_meta(i64 row0, i64 nr, i64 rowlen, i64 nsd, SortedRows 
 template <int KIND>
 static void launch_rows(bool fill, const WfnView &w, const HamView &H, i64 row0, i64 row1, i64 ncol, i64 *cf,
                         i64 *cl, int *mx, const i64 *rp, int32_t *cols, double *vals, double *diag, hipStream_t s,
-                        const SortedRows *T = nullptr) {
+                        const SortedRows *T = nullptr, u64 *bm = nullptr, i64 nchunk = 0) {
     unsigned g = grid_for(row1 - row0, ROWS_PER_BLOCK);
     if constexpr (KIND == PYCI_WFN_FULLCI) {
         if (T) {
-            k_rows<KIND, true, true><<<g, 256, 0, s>>>(w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, *T);
+            k_rows<KIND, true, true><<<g, 256, 0, s>>>(w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, *T,
+                                                       nullptr, 0);
             PE_LAUNCH_CHECK();
             return;
         }
     }
     const SortedRows none{nullptr, nullptr, nullptr, 0, 0, 0};
     if (fill)
-        k_rows<KIND, true, false><<<g, 256, 0, s>>>(w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, none);
+        k_rows<KIND, true, false><<<g, 256, 0, s>>>(w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, none,
+                                                    bm, nchunk);
     else
-        k_rows<KIND, false, false><<<g, 256, 0, s>>>(w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, none);
+        k_rows<KIND, false, false><<<g, 256, 0, s>>>(w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, none,
+                                                     bm, nchunk);
     PE_LAUNCH_CHECK();
 }
 
 static void dispatch_rows(int kind, bool fill, const WfnView &w, const HamView &H, i64 row0, i64 row1, i64 ncol,
                           i64 *cf, i64 *cl, int *mx, const i64 *rp, int32_t *cols, double *vals, double *diag,
-                          hipStream_t s) {
+                          hipStream_t s, u64 *bm, i64 nchunk) {
     if (row1 <= row0) return;
     if (kind == PYCI_WFN_DOCI)
-        launch_rows<PYCI_WFN_DOCI>(fill, w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, s);
+        launch_rows<PYCI_WFN_DOCI>(fill, w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, s, nullptr, bm,
+                                   nchunk);
     else if (kind == PYCI_WFN_FULLCI)
-        launch_rows<PYCI_WFN_FULLCI>(fill, w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, s);
+        launch_rows<PYCI_WFN_FULLCI>(fill, w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, s, nullptr, bm,
+                                     nchunk);
     else
-        launch_rows<PYCI_WFN_GENCI>(fill, w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, s);
+        launch_rows<PYCI_WFN_GENCI>(fill, w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, s, nullptr, bm,
+                                    nchunk);
+}
+
+// Chunks of 64 candidates k_rows visits per row (the same for every row of a set).
+static i64 row_chunks(int kind, i64 n, i64 nu, i64 nd) {
+    const i64 vu = n - nu, vd = n - nd, su = nu * vu, sd = nd * vd;
+    const i64 pou = nu * (nu - 1) / 2, pvu = vu * (vu - 1) / 2, pod = nd * (nd - 1) / 2, pvd = vd * (vd - 1) / 2;
+    auto ch = [](i64 x) { return (x + 63) / 64; };
+    if (kind == PYCI_WFN_GENCI) return ch(su) + pou * ch(pvu);
+    if (kind == PYCI_WFN_DOCI) return ch(su);
+    return ch(su + su * sd + pou * pvu + sd + pod * pvd);
 }
 
 // Sorted fill of rows row0 .. row1 - 1 of the full FullCI space (see SortedRows):
@@ -664,8 +689,18 @@ void build_rows(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn, i64 row0,
     PE_HIP(hipMemsetAsync(mx.p, 0, sizeof(int), s));
     PE_HIP(hipMemsetAsync(rowptr_seg.p, 0, sizeof(i64) * (nr + 1), s));
     PE_HIP(hipMemsetAsync(lowcnt_seg.p, 0, sizeof(i64) * (nr + 1), s));
+    // the hit bitmap where it fits: at most 32 GiB and a quarter of the free memory
+    const i64 nchunk = row_chunks(op->kind, wfn->nbasis, wfn->nup, op->kind == PYCI_WFN_FULLCI ? wfn->ndn : 0);
+    DevBuf<u64> bm;
+    {
+        size_t free_b = 0, total_b = 0;
+        PE_HIP(hipMemGetInfo(&free_b, &total_b));
+        const long double need = (long double)nr * (long double)nchunk * 8.0L;
+        if (nr > 0 && need <= (long double)((size_t)32 << 30) && need <= (long double)free_b / 4.0L)
+            bm.alloc((size_t)(nr * nchunk));
+    }
     dispatch_rows(op->kind, false, w, H, row0, row1, ncol, rowptr_seg.p, lowcnt_seg.p, mx.p, nullptr, nullptr,
-                  nullptr, nullptr, s);
+                  nullptr, nullptr, s, bm.p, nchunk);
     // lower-triangle total (reference size) on the host from the low counts
     {
         DevBuf<i64> tmp((size_t)(nr + 1));
@@ -680,7 +715,7 @@ void build_rows(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn, i64 row0,
     cols.alloc((size_t)nnz_seg);
     vals.alloc((size_t)nnz_seg);
     dispatch_rows(op->kind, true, w, H, row0, row1, ncol, nullptr, nullptr, nullptr, rowptr_seg.p, cols.p, vals.p,
-                  diag_seg.p, s);
+                  diag_seg.p, s, bm.p, nchunk);
     if (nr > 0) {
         k_sort_rows_lds<<<(unsigned)nr, SORT_THREADS, 0, s>>>(nr, rowptr_seg.p, cols.p, vals.p);
         PE_LAUNCH_CHECK();
@@ -689,6 +724,7 @@ void build_rows(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn, i64 row0,
             PE_LAUNCH_CHECK();
         }
     }
+    if (bm.p) PE_HIP(hipStreamSynchronize(s));  // the fill pass has read the bitmap before it is freed
 }
 
 // ---------------------------------------------------------------------------
