@@ -40,10 +40,16 @@ __device__ inline int nth_bit(u64 x, int t) {
 // occ[i] -> vir[a], 1 + ns + po npv + pv the double occ[i], occ[k] -> vir[a], vir[l]
 // with (i < k) = pair po and (a < l) = pair pv (the decode order of k_rows). The
 // fill then writes each entry at its place and no sort is needed.
+// The tables also give each excitation's colex rank (the column is rank(a') NB +
+// rank(b')) and its orbitals, packed i | a << 8 (single) or i | a << 8 | k << 16 | l << 24
+// (double, i < k occupied, a < l virtual), so the fill needs no rank computation and no
+// index decode beyond one division.
 struct SortedRows {
     const int *seg;  // [NA][La]: offset in the row of the segment of alpha excitation e
     const int *pf;   // [NB][LbF]: rank of beta excitation e among b, its singles and doubles
     const int *ps;   // [NB][LbS]: rank of beta excitation e < LbS among b and its singles
+    const int *rka, *rkb;           // [NA][La], [NB][LbF]: colex rank of the excited string
+    const unsigned *ora, *orb;      // [NA][La], [NB][LbF]: its orbitals, packed
     int La, LbF, LbS;
 };
 
@@ -109,22 +115,26 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
         if (FILL && bmrow) valid = valid && ((bmrow[chunk] >> lane) & 1ULL);
         ++chunk;
         if (valid) {
-            u64 nu = up, nd = dn;
-            if (KIND == PYCI_WFN_DOCI) {
-                nu = up ^ (1ULL << ii) ^ (1ULL << jj);
-            } else if (cls == 0) {
-                nu = up ^ (1ULL << ii) ^ (1ULL << jj);
-            } else if (cls == 1) {
-                nu = up ^ (1ULL << ii) ^ (1ULL << jj);
-                nd = dn ^ (1ULL << kk) ^ (1ULL << ll);
-            } else if (cls == 2) {
-                nu = up ^ (1ULL << ii) ^ (1ULL << jj) ^ (1ULL << kk) ^ (1ULL << ll);
-            } else if (cls == 3) {
-                nd = dn ^ (1ULL << ii) ^ (1ULL << jj);
+            if constexpr (SORTED) {
+                col = (i64)T.rka[ra * T.La + ea] * w.nsd + (i64)T.rkb[rb * T.LbF + eb];
             } else {
-                nd = dn ^ (1ULL << ii) ^ (1ULL << jj) ^ (1ULL << kk) ^ (1ULL << ll);
+                u64 nu = up, nd = dn;
+                if (KIND == PYCI_WFN_DOCI) {
+                    nu = up ^ (1ULL << ii) ^ (1ULL << jj);
+                } else if (cls == 0) {
+                    nu = up ^ (1ULL << ii) ^ (1ULL << jj);
+                } else if (cls == 1) {
+                    nu = up ^ (1ULL << ii) ^ (1ULL << jj);
+                    nd = dn ^ (1ULL << kk) ^ (1ULL << ll);
+                } else if (cls == 2) {
+                    nu = up ^ (1ULL << ii) ^ (1ULL << jj) ^ (1ULL << kk) ^ (1ULL << ll);
+                } else if (cls == 3) {
+                    nd = dn ^ (1ULL << ii) ^ (1ULL << jj);
+                } else {
+                    nd = dn ^ (1ULL << ii) ^ (1ULL << jj) ^ (1ULL << kk) ^ (1ULL << ll);
+                }
+                col = wfn_lookup(w, nu, nd);
             }
-            col = wfn_lookup(w, nu, nd);
             if (col >= ncol) col = -1;
             if (FILL && col >= 0) {
                 if (KIND == PYCI_WFN_DOCI) {
@@ -213,6 +223,46 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
                 const unsigned pv = vp[t < npv_u ? t : 0];
                 visit(t < npv_u, 2, ii, (int)(pv & 255), kk, (int)(pv >> 8), 0, 0, 0);
             }
+        }
+    } else if constexpr (SORTED) {
+        // full FullCI space: orbitals and columns from the string tables
+        const float rnsd = 1.0f / (float)(nsd > 0 ? nsd : 1);
+        for (int e0 = 0; e0 < ncand; e0 += 64) {
+            const int e = e0 + lane;
+            const bool valid = e < ncand;
+            int cls = 0, ea = 0, eb = 0, lb = 0;
+            if (valid) {
+                if (e < c0) {
+                    cls = 0;
+                    ea = 1 + e;
+                    lb = 1;
+                } else if (e < c1) {
+                    cls = 1;
+                    const int t = e - c0;
+                    int ia = (int)((float)t * rnsd);  // t / nsd, corrected below (t < 2^24)
+                    if (ia * nsd > t) --ia;
+                    else if ((ia + 1) * nsd <= t) ++ia;
+                    ea = 1 + ia;
+                    eb = 1 + (t - ia * nsd);
+                    lb = 1;
+                } else if (e < c2) {
+                    cls = 2;
+                    ea = 1 + nsu + (e - c1);
+                } else if (e < c3) {
+                    cls = 3;
+                    eb = 1 + (e - c2);
+                    lb = 2;
+                } else {
+                    cls = 4;
+                    eb = 1 + nsd + (e - c3);
+                    lb = 2;
+                }
+            }
+            const unsigned oa = T.ora[ra * T.La + ea], ob = T.orb[rb * T.LbF + eb];
+            // alpha orbitals for the classes 0-2, beta for 3-4; the 1-1 class takes both
+            const unsigned o1 = cls <= 2 ? oa : ob, o2 = cls == 1 ? (ob << 16) : o1;
+            visit(valid, cls, (int)(o1 & 255u), (int)((o1 >> 8) & 255u), (int)((o2 >> 16) & 255u),
+                  (int)((o2 >> 24) & 255u), ea, eb, lb);
         }
     } else {
         for (int e0 = 0; e0 < ncand; e0 += 64) {
@@ -458,7 +508,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_rows_global(i64 nrow, con
 constexpr int TAB_CAP = 4096;
 
 __global__ void __launch_bounds__(256) k_string_tables(int n, int nocc, const i64 *binom, int LbF, int LbS, int *seg,
-                                                       int *pf, int *ps) {
+                                                       int *pf, int *ps, int *rk, unsigned *orb) {
     __shared__ int key[TAB_CAP];
     __shared__ unsigned short pay[TAB_CAP];
     __shared__ int part[2][256];
@@ -484,19 +534,25 @@ __global__ void __launch_bounds__(256) k_string_tables(int n, int nocc, const i6
     for (int e = tid; e < N; e += 256) {
         if (e < L) {
             u64 t = str;
+            unsigned o = 0;
             if (e >= 1 && e <= ns) {
                 const int i = (e - 1) / nv, a = (e - 1) - i * nv;
-                t ^= (1ULL << nth_bit(str, i)) | (1ULL << nth_bit(vir, a));
+                const int oi = nth_bit(str, i), va = nth_bit(vir, a);
+                t ^= (1ULL << oi) | (1ULL << va);
+                o = (unsigned)oi | ((unsigned)va << 8);
             } else if (e > ns) {
                 const int d = e - 1 - ns, po = d / npv, pv = d - po * npv;
                 int i, k, a, l;
                 tri_decode(po, i, k);
                 tri_decode(pv, a, l);
-                t ^= (1ULL << nth_bit(str, i)) | (1ULL << nth_bit(str, k)) | (1ULL << nth_bit(vir, a)) |
-                     (1ULL << nth_bit(vir, l));
+                const int oi = nth_bit(str, i), ok = nth_bit(str, k), va = nth_bit(vir, a), vl = nth_bit(vir, l);
+                t ^= (1ULL << oi) | (1ULL << ok) | (1ULL << va) | (1ULL << vl);
+                o = (unsigned)oi | ((unsigned)va << 8) | ((unsigned)ok << 16) | ((unsigned)vl << 24);
             }
             key[e] = (int)colex_rank(t, binom);
             pay[e] = (unsigned short)e;
+            rk[r * L + e] = key[e];
+            orb[r * L + e] = o;
         } else {
             key[e] = 0x7fffffff;
             pay[e] = 0xffff;
@@ -589,7 +645,7 @@ static void launch_rows(bool fill, const WfnView &w, const HamView &H, i64 row0,
             return;
         }
     }
-    const SortedRows none{nullptr, nullptr, nullptr, 0, 0, 0};
+    const SortedRows none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 0};
     if (fill)
         k_rows<KIND, true, false><<<g, 256, 0, s>>>(w, H, row0, row1, ncol, cf, cl, mx, rp, cols, vals, diag, none,
                                                     bm, nchunk);
@@ -644,12 +700,15 @@ static bool build_rows_sorted(pyci_op *op, const pyci_wfn *wfn, const WfnView &w
     pyci_ctx *ctx = op->ctx;
     hipStream_t s = ctx->stream;
     DevBuf<int> seg((size_t)(NA * La)), pf((size_t)(NB * LbF)), ps((size_t)(NB * LbS));
+    DevBuf<int> rka((size_t)(NA * La)), rkb((size_t)(NB * LbF));
+    DevBuf<unsigned> ora((size_t)(NA * La)), orb((size_t)(NB * LbF));
     k_string_tables<<<(unsigned)NA, 256, 0, s>>>((int)n, (int)nu, w.binom, (int)LbF, (int)LbS, seg.p, nullptr,
-                                                 nullptr);
+                                                 nullptr, rka.p, ora.p);
     PE_LAUNCH_CHECK();
-    k_string_tables<<<(unsigned)NB, 256, 0, s>>>((int)n, (int)nd, w.binom, (int)LbF, (int)LbS, nullptr, pf.p, ps.p);
+    k_string_tables<<<(unsigned)NB, 256, 0, s>>>((int)n, (int)nd, w.binom, (int)LbF, (int)LbS, nullptr, pf.p, ps.p,
+                                                 rkb.p, orb.p);
     PE_LAUNCH_CHECK();
-    const SortedRows T{seg.p, pf.p, ps.p, (int)La, (int)LbF, (int)LbS};
+    const SortedRows T{seg.p, pf.p, ps.p, rka.p, rkb.p, ora.p, orb.p, (int)La, (int)LbF, (int)LbS};
     rowptr_seg.alloc((size_t)(nr + 1));
     lowcnt_seg.alloc((size_t)(nr + 1));
     diag_seg.alloc((size_t)(nr > 0 ? nr : 1));
