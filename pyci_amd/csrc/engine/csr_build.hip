@@ -12,6 +12,8 @@
 // for the lower-triangle export), an exclusive scan turns counts into row
 // pointers, pass 2 writes entries (wave-level ballot compaction) and the
 // diagonal, and a per-row bitonic sort in LDS orders each row by column.
+#include <hipcub/hipcub.hpp>
+
 #include "internal.hpp"
 
 #pragma clang fp contract(off)
@@ -389,21 +391,26 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
 }
 
 // ---------------------------------------------------------------------------
-// Per-row sort by column (sort_row, sparseop.cpp:214-218). Bitonic network in
-// the all-ascending form, so padding past the row end behaves as +infinity.
+// Per-row sort by column (sort_row, sparseop.cpp:214-218). Rows of up to 4,096
+// entries: a bitonic network in LDS in the all-ascending form, so padding past the
+// row end behaves as +infinity, with the LDS sized for CAP entries (rows of at most
+// 1,024 entries, most rows of a selected set, in the 12 KB variant, so that more
+// blocks are resident). Longer rows: a segmented radix sort over a gathered copy.
 // ---------------------------------------------------------------------------
 constexpr int SORT_CAP = 4096;
 constexpr int SORT_THREADS = 256;
 
+// one block per row; rows with lo < len <= CAP
+template <int CAP>
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_rows_lds(i64 nrow, const i64 *rowptr, int32_t *cols,
-                                                                double *vals) {
-    __shared__ int32_t sk[SORT_CAP];
-    __shared__ double sv[SORT_CAP];
+                                                                double *vals, int lo) {
+    __shared__ int32_t sk[CAP];
+    __shared__ double sv[CAP];
     const i64 row = blockIdx.x;
     if (row >= nrow) return;
     const i64 b = rowptr[row];
     const int len = (int)(rowptr[row + 1] - b);
-    if (len <= 1 || len > SORT_CAP) return;  // uniform per block
+    if (len <= lo || len <= 1 || len > CAP) return;  // uniform per block
     int N = 1;
     while (N < len) N <<= 1;
     for (int t = threadIdx.x; t < N; t += SORT_THREADS) {
@@ -452,49 +459,66 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_rows_lds(i64 nrow, const 
     }
 }
 
-// Rows longer than SORT_CAP: the same network directly on global memory.
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_rows_global(i64 nrow, const i64 *rowptr, int32_t *cols,
-                                                                   double *vals) {
-    const i64 row = blockIdx.x;
-    if (row >= nrow) return;
-    const i64 b = rowptr[row];
-    const i64 len = rowptr[row + 1] - b;
+// Rows longer than SORT_CAP: their ids (in no particular order) and entry count.
+__global__ void k_long_rows(i64 nrow, const i64 *rowptr, unsigned long long *cnt, i64 *ids) {
+    const i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    const i64 len = rowptr[r + 1] - rowptr[r];
     if (len <= SORT_CAP) return;
-    int32_t *k = cols + b;
-    double *v = vals + b;
-    i64 N = 1;
-    while (N < len) N <<= 1;
-    for (i64 size = 2; size <= N; size <<= 1) {
-        const i64 half = size >> 1;
-        for (i64 t = threadIdx.x; t < N / 2; t += SORT_THREADS) {
-            i64 blk = t / half, o = t - blk * half;
-            i64 p = blk * size + o, q = blk * size + size - 1 - o;
-            if (q < len && k[q] < k[p]) {
-                int32_t tk = k[p];
-                k[p] = k[q];
-                k[q] = tk;
-                double tv = v[p];
-                v[p] = v[q];
-                v[q] = tv;
-            }
-        }
-        __syncthreads();
-        for (i64 stride = half >> 1; stride > 0; stride >>= 1) {
-            for (i64 t = threadIdx.x; t < N / 2; t += SORT_THREADS) {
-                i64 blk = t / stride, o = t - blk * stride;
-                i64 p = blk * 2 * stride + o, q = p + stride;
-                if (q < len && k[q] < k[p]) {
-                    int32_t tk = k[p];
-                    k[p] = k[q];
-                    k[q] = tk;
-                    double tv = v[p];
-                    v[p] = v[q];
-                    v[q] = tv;
-                }
-            }
-            __syncthreads();
+    const unsigned long long o = atomicAdd(cnt, 1ULL);
+    atomicAdd(cnt + 1, (unsigned long long)len);
+    ids[o] = r;
+}
+
+__global__ void k_long_lens(const i64 *ids, i64 L, const i64 *rowptr, i64 *off) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= L) off[i] = (i < L) ? rowptr[ids[i] + 1] - rowptr[ids[i]] : 0;
+}
+
+// one block per long row: to (dir 0) or from (dir 1) the gathered copy
+__global__ void k_long_move(const i64 *ids, const i64 *off, const i64 *rowptr, int32_t *cols, double *vals,
+                            int32_t *gk, double *gv, int dir) {
+    const i64 i = blockIdx.x, b = rowptr[ids[i]], g = off[i], len = off[i + 1] - g;
+    for (i64 t = threadIdx.x; t < len; t += blockDim.x) {
+        if (dir == 0) {
+            gk[g + t] = cols[b + t];
+            gv[g + t] = vals[b + t];
+        } else {
+            cols[b + t] = gk[g + t];
+            vals[b + t] = gv[g + t];
         }
     }
+}
+
+static void sort_long_rows(pyci_ctx *ctx, i64 nr, const i64 *rowptr, int32_t *cols, double *vals, hipStream_t s) {
+    DevBuf<unsigned long long> cnt(2);
+    DevBuf<i64> ids((size_t)nr);
+    PE_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    k_long_rows<<<grid_for(nr, 256), 256, 0, s>>>(nr, rowptr, cnt.p, ids.p);
+    PE_LAUNCH_CHECK();
+    unsigned long long h[2] = {0, 0};
+    PE_HIP(hipMemcpyAsync(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    const i64 L = (i64)h[0], M = (i64)h[1];
+    if (L == 0) return;
+    PE_REQUIRE(M < ((i64)1 << 31) && L < ((i64)1 << 31), PYCI_EUNSUPPORTED, "rows too long to sort");
+    DevBuf<i64> off((size_t)(L + 1));
+    k_long_lens<<<grid_for(L + 1, 256), 256, 0, s>>>(ids.p, L, rowptr, off.p);
+    PE_LAUNCH_CHECK();
+    exclusive_scan(ctx, off.p, L + 1);
+    DevBuf<int32_t> k0((size_t)M), k1((size_t)M);
+    DevBuf<double> v0((size_t)M), v1((size_t)M);
+    k_long_move<<<(unsigned)L, 256, 0, s>>>(ids.p, off.p, rowptr, cols, vals, k0.p, v0.p, 0);
+    PE_LAUNCH_CHECK();
+    size_t tmp_bytes = 0;
+    PE_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tmp_bytes, k0.p, k1.p, v0.p, v1.p, (int)M, (int)L,
+                                                       off.p, off.p + 1, 0, 32, s));
+    DevBuf<unsigned char> tmp(tmp_bytes);
+    PE_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.p, tmp_bytes, k0.p, k1.p, v0.p, v1.p, (int)M, (int)L,
+                                                       off.p, off.p + 1, 0, 32, s));
+    k_long_move<<<(unsigned)L, 256, 0, s>>>(ids.p, off.p, rowptr, cols, vals, k1.p, v1.p, 1);
+    PE_LAUNCH_CHECK();
+    PE_HIP(hipStreamSynchronize(s));  // the gathered copies leave scope
 }
 
 // ---------------------------------------------------------------------------
@@ -776,12 +800,13 @@ void build_rows(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn, i64 row0,
     dispatch_rows(op->kind, true, w, H, row0, row1, ncol, nullptr, nullptr, nullptr, rowptr_seg.p, cols.p, vals.p,
                   diag_seg.p, s, bm.p, nchunk);
     if (nr > 0) {
-        k_sort_rows_lds<<<(unsigned)nr, SORT_THREADS, 0, s>>>(nr, rowptr_seg.p, cols.p, vals.p);
+        k_sort_rows_lds<1024><<<(unsigned)nr, SORT_THREADS, 0, s>>>(nr, rowptr_seg.p, cols.p, vals.p, 1);
         PE_LAUNCH_CHECK();
-        if (hmax > SORT_CAP) {
-            k_sort_rows_global<<<(unsigned)nr, SORT_THREADS, 0, s>>>(nr, rowptr_seg.p, cols.p, vals.p);
+        if (hmax > 1024) {
+            k_sort_rows_lds<SORT_CAP><<<(unsigned)nr, SORT_THREADS, 0, s>>>(nr, rowptr_seg.p, cols.p, vals.p, 1024);
             PE_LAUNCH_CHECK();
         }
+        if (hmax > SORT_CAP) sort_long_rows(ctx, nr, rowptr_seg.p, cols.p, vals.p, s);
     }
     if (bm.p) PE_HIP(hipStreamSynchronize(s));  // the fill pass has read the bitmap before it is freed
 }
