@@ -565,7 +565,7 @@ def main():
     fp32 = 1 if args.precision == "mixed" else 0
     tparts = parts(op, nroot, 20 if nroot == 1 else 3, fp32)
     L.pyci_op_destroy(op)
-    names = ["k_spmv<4> (CSR sigma)"] if storage == "csr" else [
+    names = ["k_spmv (CSR sigma)"] if storage == "csr" else [
         "k_mf_dprod (gather + pair-integral product, MFMA)", "k_mf_sigma (scatter pass)"]
     for nm, v in zip(names, tparts):
         log(f"{storage} sigma part {nm}: {v['ms']:.3f} ms, {v['bytes'] / v['ms'] / 1e6:.0f} GB/s, "
