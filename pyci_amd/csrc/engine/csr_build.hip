@@ -475,6 +475,12 @@ __global__ void k_long_lens(const i64 *ids, i64 L, const i64 *rowptr, i64 *off) 
     if (i <= L) off[i] = (i < L) ? rowptr[ids[i] + 1] - rowptr[ids[i]] : 0;
 }
 
+// off[0 .. n] - base (offsets of one batch of long rows, relative to the batch)
+__global__ void k_long_rebase(const i64 *off, i64 n, i64 base, i64 *out) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) out[i] = off[i] - base;
+}
+
 // one block per long row: to (dir 0) or from (dir 1) the gathered copy
 __global__ void k_long_move(const i64 *ids, const i64 *off, const i64 *rowptr, int32_t *cols, double *vals,
                             int32_t *gk, double *gv, int dir) {
@@ -490,6 +496,11 @@ __global__ void k_long_move(const i64 *ids, const i64 *off, const i64 *rowptr, i
     }
 }
 
+// Rows longer than SORT_CAP, in batches of at most kLongBatch entries (a row is
+// shorter than the 2^31 columns of the operator): gather, segmented radix sort,
+// scatter back.
+constexpr i64 kLongBatch = (i64)1 << 27;
+
 static void sort_long_rows(pyci_ctx *ctx, i64 nr, const i64 *rowptr, int32_t *cols, double *vals, hipStream_t s) {
     DevBuf<unsigned long long> cnt(2);
     DevBuf<i64> ids((size_t)nr);
@@ -499,25 +510,48 @@ static void sort_long_rows(pyci_ctx *ctx, i64 nr, const i64 *rowptr, int32_t *co
     unsigned long long h[2] = {0, 0};
     PE_HIP(hipMemcpyAsync(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
-    const i64 L = (i64)h[0], M = (i64)h[1];
+    const i64 L = (i64)h[0];
     if (L == 0) return;
-    PE_REQUIRE(M < ((i64)1 << 31) && L < ((i64)1 << 31), PYCI_EUNSUPPORTED, "rows too long to sort");
     DevBuf<i64> off((size_t)(L + 1));
     k_long_lens<<<grid_for(L + 1, 256), 256, 0, s>>>(ids.p, L, rowptr, off.p);
     PE_LAUNCH_CHECK();
     exclusive_scan(ctx, off.p, L + 1);
-    DevBuf<int32_t> k0((size_t)M), k1((size_t)M);
-    DevBuf<double> v0((size_t)M), v1((size_t)M);
-    k_long_move<<<(unsigned)L, 256, 0, s>>>(ids.p, off.p, rowptr, cols, vals, k0.p, v0.p, 0);
-    PE_LAUNCH_CHECK();
-    size_t tmp_bytes = 0;
-    PE_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tmp_bytes, k0.p, k1.p, v0.p, v1.p, (int)M, (int)L,
-                                                       off.p, off.p + 1, 0, 32, s));
-    DevBuf<unsigned char> tmp(tmp_bytes);
-    PE_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.p, tmp_bytes, k0.p, k1.p, v0.p, v1.p, (int)M, (int)L,
-                                                       off.p, off.p + 1, 0, 32, s));
-    k_long_move<<<(unsigned)L, 256, 0, s>>>(ids.p, off.p, rowptr, cols, vals, k1.p, v1.p, 1);
-    PE_LAUNCH_CHECK();
+    std::vector<i64> hoff((size_t)(L + 1));
+    PE_HIP(hipMemcpyAsync(hoff.data(), off.p, sizeof(i64) * (L + 1), hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    i64 cap = 0;  // the largest batch
+    for (i64 b = 0; b < L;) {
+        i64 e = b + 1;
+        while (e < L && hoff[(size_t)(e + 1)] - hoff[(size_t)b] <= kLongBatch) ++e;
+        cap = std::max(cap, hoff[(size_t)e] - hoff[(size_t)b]);
+        b = e;
+    }
+    PE_REQUIRE(cap < ((i64)1 << 31), PYCI_EUNSUPPORTED, "row too long to sort");
+    DevBuf<int32_t> k0((size_t)cap), k1((size_t)cap);
+    DevBuf<double> v0((size_t)cap), v1((size_t)cap);
+    DevBuf<i64> loc((size_t)(L + 1));
+    DevBuf<unsigned char> tmp;
+    for (i64 b = 0; b < L;) {
+        i64 e = b + 1;
+        while (e < L && hoff[(size_t)(e + 1)] - hoff[(size_t)b] <= kLongBatch) ++e;
+        const i64 nrow_b = e - b, M = hoff[(size_t)e] - hoff[(size_t)b];
+        k_long_rebase<<<grid_for(nrow_b + 1, 256), 256, 0, s>>>(off.p + b, nrow_b, hoff[(size_t)b], loc.p);
+        PE_LAUNCH_CHECK();
+        k_long_move<<<(unsigned)nrow_b, 256, 0, s>>>(ids.p + b, loc.p, rowptr, cols, vals, k0.p, v0.p, 0);
+        PE_LAUNCH_CHECK();
+        size_t tmp_bytes = 0;
+        PE_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tmp_bytes, k0.p, k1.p, v0.p, v1.p, (int)M,
+                                                           (int)nrow_b, loc.p, loc.p + 1, 0, 32, s));
+        if (tmp.n < tmp_bytes) {
+            PE_HIP(hipStreamSynchronize(s));  // the previous batch may still use the old buffer
+            tmp.alloc(tmp_bytes);
+        }
+        PE_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.p, tmp_bytes, k0.p, k1.p, v0.p, v1.p, (int)M,
+                                                           (int)nrow_b, loc.p, loc.p + 1, 0, 32, s));
+        k_long_move<<<(unsigned)nrow_b, 256, 0, s>>>(ids.p + b, loc.p, rowptr, cols, vals, k1.p, v1.p, 1);
+        PE_LAUNCH_CHECK();
+        b = e;
+    }
     PE_HIP(hipStreamSynchronize(s));  // the gathered copies leave scope
 }
 
