@@ -806,15 +806,16 @@ void build_rows(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn, i64 row0,
     PE_HIP(hipMemsetAsync(mx.p, 0, sizeof(int), s));
     PE_HIP(hipMemsetAsync(rowptr_seg.p, 0, sizeof(i64) * (nr + 1), s));
     PE_HIP(hipMemsetAsync(lowcnt_seg.p, 0, sizeof(i64) * (nr + 1), s));
-    // the hit bitmap where it fits: at most 32 GiB and a quarter of the free memory
+    // the hit bitmap where it fits in a third of the free memory (36.6 GB at the 6.9 M
+    // determinants of the GenCI selection workload); should the CSR arrays then not
+    // fit beside it, it is released and the fill pass enumerates every candidate again
     const i64 nchunk = row_chunks(op->kind, wfn->nbasis, wfn->nup, op->kind == PYCI_WFN_FULLCI ? wfn->ndn : 0);
     DevBuf<u64> bm;
     {
         size_t free_b = 0, total_b = 0;
         PE_HIP(hipMemGetInfo(&free_b, &total_b));
         const long double need = (long double)nr * (long double)nchunk * 8.0L;
-        if (nr > 0 && need <= (long double)((size_t)32 << 30) && need <= (long double)free_b / 4.0L)
-            bm.alloc((size_t)(nr * nchunk));
+        if (nr > 0 && need <= (long double)free_b / 3.0L) bm.alloc((size_t)(nr * nchunk));
     }
     dispatch_rows(op->kind, false, w, H, row0, row1, ncol, rowptr_seg.p, lowcnt_seg.p, mx.p, nullptr, nullptr,
                   nullptr, nullptr, s, bm.p, nchunk);
@@ -829,8 +830,16 @@ void build_rows(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn, i64 row0,
     PE_HIP(hipMemcpyAsync(&hmax, mx.p, sizeof(int), hipMemcpyDeviceToHost, s));
     PE_HIP(hipStreamSynchronize(s));
     PE_REQUIRE(nnz_seg < (i64)1 << 40, PYCI_ENOMEM, "sparse operator too large");
-    cols.alloc((size_t)nnz_seg);
-    vals.alloc((size_t)nnz_seg);
+    try {
+        cols.alloc((size_t)nnz_seg);
+        vals.alloc((size_t)nnz_seg);
+    } catch (const EngineError &e) {
+        if (e.code != PYCI_ENOMEM || !bm.p) throw;
+        (void)hipGetLastError();  // the failed allocation must not be reported by a later launch check
+        bm.release();
+        cols.alloc((size_t)nnz_seg);
+        vals.alloc((size_t)nnz_seg);
+    }
     dispatch_rows(op->kind, true, w, H, row0, row1, ncol, nullptr, nullptr, nullptr, rowptr_seg.p, cols.p, vals.p,
                   diag_seg.p, s, bm.p, nchunk);
     if (nr > 0) {

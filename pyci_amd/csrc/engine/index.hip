@@ -33,6 +33,8 @@ WfnView view_of(const pyci_wfn *w) {
     v.hkeys = w->hkeys.p;
     v.hvals = w->hvals.p;
     v.htags = w->htags.p;
+    v.hfilt = w->hfilt.p;
+    v.fmask = w->fbits ? (u64)(w->fbits - 1) : 0;
     v.hmask = w->hcap ? (u64)(w->hcap - 1) : 0;
     return v;
 }
@@ -74,11 +76,15 @@ __global__ void k_hash_clear(i64 *vals, u64 *tags, i64 n) {
 // is claimed by CAS on its value and the key is written by the claimer; no
 // lookups run concurrently with this kernel.
 __global__ void k_hash_insert(const u64 *dets, int nkey, i64 first, i64 ndet, u64 *keys, i64 *vals, u64 *tags,
-                              u64 mask) {
+                              u64 mask, unsigned *filt, u64 fmask) {
     i64 i = first + (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ndet) return;
     u64 a = dets[nkey * i], b = (nkey == 2) ? dets[nkey * i + 1] : 0ULL;
     const u64 hk = hash_key(a, b);
+    if (filt) {
+        const u64 f = (hk >> 24) & fmask;
+        atomicOr(&filt[f >> 5], 1u << (f & 31));
+    }
     u64 h = hk & mask;
     for (;;) {
         unsigned long long old = atomicCAS((unsigned long long *)&vals[h], (unsigned long long)(-1LL),
@@ -128,7 +134,9 @@ void wfn_build_index(pyci_wfn *w) {
         w->hkeys.release();
         w->hvals.release();
         w->htags.release();
+        w->hfilt.release();
         w->hcap = 0;
+        w->fbits = 0;
         return;
     }
     // 2) hash table at load factor <= 1/2
@@ -140,9 +148,19 @@ void wfn_build_index(pyci_wfn *w) {
     w->htags.alloc((size_t)(cap / 8));
     k_hash_clear<<<grid_for(cap, 256), 256, 0, s>>>(w->hvals.p, w->htags.p, cap);
     PE_LAUNCH_CHECK();
+    // the presence filter once the tag array outgrows an XCD's 4 MB L2: 2 bits per slot,
+    // at least 4 per determinant (the table's load is at most 1/2), 1/8 the tags' bytes
+    w->hfilt.release();
+    w->fbits = 0;
+    if (cap >= ((i64)1 << 22)) {
+        w->fbits = 2 * cap;
+        w->hfilt.alloc((size_t)(w->fbits / 32));
+        PE_HIP(hipMemsetAsync(w->hfilt.p, 0, sizeof(unsigned) * (size_t)(w->fbits / 32), s));
+    }
     if (w->ndet > 0) {
         k_hash_insert<<<grid_for(w->ndet, 256), 256, 0, s>>>(w->dets.p, w->nkey, 0, w->ndet, w->hkeys.p, w->hvals.p,
-                                                              w->htags.p, (u64)(cap - 1));
+                                                              w->htags.p, (u64)(cap - 1), w->hfilt.p,
+                                                              w->fbits ? (u64)(w->fbits - 1) : 0);
         PE_LAUNCH_CHECK();
     }
 }
@@ -158,7 +176,8 @@ void wfn_index_append(pyci_wfn *w, i64 first) {
     i64 n = w->ndet - first;
     if (n > 0) {
         k_hash_insert<<<grid_for(n, 256), 256, 0, s>>>(w->dets.p, w->nkey, first, w->ndet, w->hkeys.p, w->hvals.p,
-                                                       w->htags.p, (u64)(w->hcap - 1));
+                                                       w->htags.p, (u64)(w->hcap - 1), w->hfilt.p,
+                                                       w->fbits ? (u64)(w->fbits - 1) : 0);
         PE_LAUNCH_CHECK();
     }
 }

@@ -179,6 +179,10 @@ struct pyci_wfn {
     pe::DevBuf<pe::i64> hvals;
     pe::DevBuf<pe::u64> htags;
     pe::i64 hcap = 0;
+    // large tables only: a presence filter, one bit per determinant's filter hash in an
+    // array of fbits bits (about 4 per determinant), for lookups that miss
+    pe::DevBuf<unsigned> hfilt;
+    pe::i64 fbits = 0;
 };
 
 namespace pe {
@@ -253,7 +257,8 @@ struct WfnView {
     const u64 *hkeys;
     const i64 *hvals;
     const u64 *htags;
-    u64 hmask;
+    const unsigned *hfilt;  // or null
+    u64 hmask, fmask;
 };
 
 struct HamView {
@@ -309,6 +314,11 @@ __device__ inline i64 wfn_lookup(const WfnView &w, u64 a, u64 b) {
     // exactly the zero bytes; applied to x ^ tag it flags the matching bytes (and at
     // most spurious bytes above a true match, which the key compare rejects).
     const u64 hk = hash_key(a, b), ones = 0x0101010101010101ULL, highs = 0x8080808080808080ULL;
+    if (w.hfilt) {
+        // bits 24..55 of the hash: independent of the slot (low bits) and the tag (top 7)
+        const u64 f = (hk >> 24) & w.fmask;
+        if (!((w.hfilt[f >> 5] >> (f & 31)) & 1u)) return -1;
+    }
     const u64 tagw = ((hk >> 57) | 0x80ULL) * ones;
     u64 h = hk & w.hmask;
     for (;;) {

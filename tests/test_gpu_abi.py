@@ -103,3 +103,40 @@ def test_abi_rejects_bad_shapes(eng):
     dets = np.zeros(2, dtype=np.uint64)
     wfn = ctypes.c_void_p()
     assert L.pyci_wfn_create(ctx, 1, 4, 1, 2, 1, p(dets, ctypes.c_uint64), ctypes.byref(wfn)) == 1
+
+
+def test_abi_large_hashed_index(eng):
+    """A hashed set large enough for the index's presence filter (tables of 2^22+
+    slots): FullCI nbasis 13, 6a+6b without 1,000 of its 2,944,656 determinants and
+    in a shuffled order. pyci_wfn_index (onespinwfn.cpp:123-157 index_det) finds
+    every member at its position and none of the left-out ones, nor determinants
+    with other electron counts."""
+    L, ctx = eng
+    nd = 1716 * 1716
+    dets = np.zeros((nd, 2), dtype=np.uint64)
+    assert L.pyci_fill_all_dets(ctx, 1, 13, 6, 6, p(dets, ctypes.c_uint64)) == 0
+    rng = np.random.default_rng(21)
+    perm = rng.permutation(nd)
+    keep, out_ = perm[1000:], perm[:1000]
+    mine = np.ascontiguousarray(dets[keep])
+    wfn = ctypes.c_void_p()
+    assert L.pyci_wfn_create(ctx, 1, 13, 6, 6, mine.shape[0], p(mine, ctypes.c_uint64), ctypes.byref(wfn)) == 0, \
+        L.pyci_last_error()
+    try:
+        got = np.zeros(mine.shape[0], np.int64)
+        assert L.pyci_wfn_index(wfn, mine.shape[0], p(mine, ctypes.c_uint64), p(got, ctypes.c_int64)) == 0
+        npt.assert_array_equal(got, np.arange(mine.shape[0]))
+        absent = np.ascontiguousarray(dets[out_])
+        got = np.zeros(1000, np.int64)
+        assert L.pyci_wfn_index(wfn, 1000, p(absent, ctypes.c_uint64), p(got, ctypes.c_int64)) == 0
+        assert np.all(got == -1)
+        # alpha string with 7 electrons: never a member
+        bad = absent.copy()
+        bad[:, 0] |= np.uint64(1) << np.uint64(12)
+        bad = bad[(bad[:, 0] != absent[:, 0])]
+        got = np.zeros(bad.shape[0], np.int64)
+        assert L.pyci_wfn_index(wfn, bad.shape[0], p(np.ascontiguousarray(bad), ctypes.c_uint64),
+                                p(got, ctypes.c_int64)) == 0
+        assert np.all(got == -1)
+    finally:
+        L.pyci_wfn_destroy(wfn)
