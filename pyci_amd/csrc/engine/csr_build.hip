@@ -9,9 +9,12 @@
 // order, so values are bit-identical to the serial code.
 //
 // Pass 1 counts entries per row (all stored entries, and those with col <= row
-// for the lower-triangle export), an exclusive scan turns counts into row
-// pointers, pass 2 writes entries (wave-level ballot compaction) and the
-// diagonal, and a per-row bitonic sort in LDS orders each row by column.
+// for the lower-triangle export) and keeps each chunk's hit ballot, an exclusive
+// scan turns counts into row pointers, pass 2 writes entries (wave-level ballot
+// compaction, probing only the hits) and the diagonal, and each row is sorted by
+// column (bitonic in LDS up to 4,096 entries, a segmented radix sort beyond).
+// The full FullCI space in colex order skips the count pass and the sort: its
+// rows are written in column order from per-string tables (SortedRows).
 #include <hipcub/hipcub.hpp>
 
 #include "internal.hpp"
