@@ -20,6 +20,7 @@
 // CGS2 against the basis + Cholesky-QR of the new block -> sigma of the new
 // columns (FP32 or FP64 per problem) -> new Rayleigh-matrix entries.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -1443,8 +1444,8 @@ static void single_solve(pyci_ctx *ctx, int n, int na, int nb, double ecore, con
     *ns = st.sigma_calls;
 }
 
-// Problems of at least kSingleND determinants take the single-problem solver, one
-// after another from the largest down on a host thread of their own, while the
+// Problems of at least kSingleND determinants take the single-problem solver, on two
+// workers that take them from the largest down, on host threads of their own, while the
 // smaller ones run together as the replayed graph on a stream of their own. The
 // graph's iteration costs the sum of every active problem's work, so a large
 // problem in it sets the pace for all; alone, it runs at the single solver's speed
@@ -1496,29 +1497,67 @@ void batch_solve(pyci_ctx *ctx, i64 nprob, const i64 *nbasis, const i64 *nup, co
     std::vector<int> bconv(big.size());
     const int device = ctx->device;
     std::exception_ptr err_big;
+    // The large problems go to two workers, each pulling the next largest: the second
+    // on a context of its own (streams, workspace, solver memory), so that one
+    // solve's kernels fill the GPU while the other waits for its host's Rayleigh-Ritz
+    // steps (the problems of ~200,000 determinants leave it idle otherwise).
+    std::atomic<size_t> next{0};
+    unsigned long long t0 = 0;
+    int rate_khz = 0;
+    auto worker = [&](pyci_ctx *c) {
+        DevBuf<unsigned long long> stamp(1);
+        auto now = [&] {
+            unsigned long long t = 0;
+            mf_wallclock_launch(stamp.p, c->stream);
+            PE_HIP(hipMemcpyAsync(&t, stamp.p, sizeof(t), hipMemcpyDeviceToHost, c->stream));
+            PE_HIP(hipStreamSynchronize(c->stream));
+            return t;
+        };
+        for (size_t i = next++; i < big.size(); i = next++) {
+            const i64 p = big[i];
+            single_solve(c, (int)nbasis[p], (int)nup[p], (int)ndn[p], ecore[p], one_mo[p], two_mo[p], nroot, tol,
+                         precision, maxiter, &bev[i * (size_t)nroot], &bit[i], &bns[i], &bconv[i]);
+            blat[i] = rate_khz > 0 ? (double)(now() - t0) / (double)rate_khz : -1.0;
+        }
+    };
     std::thread tb([&] {
+        pyci_ctx *c2 = nullptr;
+        std::exception_ptr err2;
         try {
             PE_HIP(hipSetDevice(device));
-            int rate_khz = 0;
             PE_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device));
-            DevBuf<unsigned long long> stamp(1);
-            auto now = [&] {
-                unsigned long long t = 0;
+            {
+                DevBuf<unsigned long long> stamp(1);
                 mf_wallclock_launch(stamp.p, ctx->stream);
-                PE_HIP(hipMemcpyAsync(&t, stamp.p, sizeof(t), hipMemcpyDeviceToHost, ctx->stream));
+                PE_HIP(hipMemcpyAsync(&t0, stamp.p, sizeof(t0), hipMemcpyDeviceToHost, ctx->stream));
                 PE_HIP(hipStreamSynchronize(ctx->stream));
-                return t;
-            };
-            const unsigned long long t0 = now();
-            for (size_t i = 0; i < big.size(); ++i) {
-                const i64 p = big[i];
-                single_solve(ctx, (int)nbasis[p], (int)nup[p], (int)ndn[p], ecore[p], one_mo[p], two_mo[p], nroot,
-                             tol, precision, maxiter, &bev[i * (size_t)nroot], &bit[i], &bns[i], &bconv[i]);
-                blat[i] = rate_khz > 0 ? (double)(now() - t0) / (double)rate_khz : -1.0;
             }
+            std::thread t2;
+            if (big.size() > 1) {
+                if (pyci_ctx_create(device, &c2) != PYCI_OK) throw EngineError(PYCI_EHIP, pyci_last_error());
+                t2 = std::thread([&] {
+                    try {
+                        PE_HIP(hipSetDevice(device));
+                        worker(c2);
+                    } catch (...) {
+                        err2 = std::current_exception();
+                        next = big.size();  // the other worker stops after its current problem
+                    }
+                });
+            }
+            try {
+                worker(ctx);
+            } catch (...) {
+                next = big.size();
+                if (t2.joinable()) t2.join();
+                throw;
+            }
+            if (t2.joinable()) t2.join();
         } catch (...) {
             err_big = std::current_exception();
         }
+        if (c2) pyci_ctx_destroy(c2);
+        if (!err_big && err2) err_big = err2;
     });
     hipStream_t gs = nullptr;
     pyci_batch_stats gst{};
