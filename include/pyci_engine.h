@@ -291,8 +291,9 @@ int pyci_fanci_jacobian(pyci_fanci *f, const double *x, double *y);
  * 114-146) on the full FullCI space of its own integrals (add_all_dets). The
  * problems below 150,000 determinants run together on the device (matrix-free
  * sigma, one kernel launch per Davidson step over all of them, the iteration
- * replayed as a hipGraph); the larger ones, each of which fills the GPU by
- * itself, are solved one after another by the single-problem solver beside them.
+ * replayed as a hipGraph); the larger ones are solved by the single-problem
+ * solver beside them, two at a time from the largest down (the second on a
+ * context the call creates and destroys).
  * ------------------------------------------------------------------------ */
 typedef struct pyci_batch_stats {
     int64_t graph_launches; /* Davidson iterations replayed (the graph's maximum) */
