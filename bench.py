@@ -565,8 +565,12 @@ def main():
     fp32 = 1 if args.precision == "mixed" else 0
     tparts = parts(op, nroot, 20 if nroot == 1 else 3, fp32)
     L.pyci_op_destroy(op)
+    # one vector with nup == ndn: the parity-sector sigma the solver runs (live-row product,
+    # alpha-term scatter + symmetrisation)
+    sector = storage == "direct" and nroot == 1 and nu == nd_
     names = ["k_spmv (CSR sigma)"] if storage == "csr" else [
-        "k_mf_dprod (gather + pair-integral product, MFMA)", "k_mf_sigma (scatter pass)"]
+        "k_mf_dprod (gather + pair-integral product, MFMA)",
+        "k_mf_alpha + k_mf_symm_tiles (parity-sector scatter)" if sector else "k_mf_sigma (scatter pass)"]
     for nm, v in zip(names, tparts):
         log(f"{storage} sigma part {nm}: {v['ms']:.3f} ms, {v['bytes'] / v['ms'] / 1e6:.0f} GB/s, "
             f"{v['flops'] / v['ms'] / 1e9:.1f} TFLOP/s")

@@ -118,6 +118,9 @@ enum { VM_GLOBAL = 0, VM_FULL = 1 };
 
 // pair count implied by a pair dimension of kpad = 16 MT >= 96 (else 0: not fixed)
 __host__ __device__ constexpr int fixed_npair(int MT) { return MT == 6 ? 91 : MT == 7 ? 105 : MT == 8 ? 120 : 0; }
+// row tiles of the fused pass that hold a pair, for a pair dimension of 16 MT
+__host__ __device__ constexpr int pair_tiles(int MT) { return fixed_npair(MT) ? (fixed_npair(MT) + 15) / 16 : MT; }
+
 // k extent of the product (a multiple of 4); the sink row of the gather buffers is row VC
 __host__ __device__ constexpr int dprod_vc(int MT) {
     return fixed_npair(MT) ? 4 * ((fixed_npair(MT) + 3) / 4) : 16 * MT;
@@ -134,9 +137,15 @@ struct DprodArgs {
     int offb;  // start of the beta entries in the merged table, NA * nea (entry t of beta string r at offb + t NB + r)
     int sent;  // index of the sentinel entry
     int ld;
+    // live-row layout (ML > 0): column c = Ja * nbp + Jb (nbp: NB rounded up to 64, so a
+    // tile holds one alpha string; columns with Jb >= NB are empty), output row r of a
+    // tile of alpha string Ka is pair live[Ka * 16 ML + r] (r < nl; pad rows name a zero
+    // row of V)
+    int nbp, nl;
+    const int *live;
 };
 
-template <int MT, typename T, bool SB, int VM, int PE>
+template <int MT, typename T, bool SB, int VM, int PE, int ML>
 __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__restrict__ tab, const T *__restrict__ X0,
                                                   const T *__restrict__ V, const T *__restrict__ kv,
                                                   T *__restrict__ W) {
@@ -151,6 +160,9 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
     constexpr int NP = fixed_npair(MT);             // 0: pair count known only at run time
     constexpr int VC = dprod_vc(MT), NQ = VC / 4;   // k extent and k steps
     constexpr int MTL = NP ? (NP + 15) / 16 : MT;   // row tiles that hold a pair
+    constexpr int MR = ML ? ML : MTL;               // output row tiles
+    constexpr int RS = ML ? 16 * ML : KP;           // rows of a tile's block of W
+    static_assert(ML == 0 || ML < MTL, "the live rows must be fewer than the pair rows");
     constexpr int BR = VC + 1, LS = VC + 1, NBUF = SB ? 1 : 2;
     static_assert(!(SB && VM == VM_GLOBAL), "one buffer only with V in LDS");
     // With V in LDS and a free padding row VC - 1 in the k extent (every pair
@@ -181,10 +193,23 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
         // index of an alpha entry is en.x * sa + na, of a beta entry en.x * sb + nb
         auto fetch = [&](int step, int2 *nx, int &na, int &nb) {
             const int c0 = ((int)blockIdx.x + step * (int)gridDim.x) * 64 + lane;
-            const bool live = step < nmine && c0 < A.ncol;
+            bool live = step < nmine && c0 < A.ncol;
             const unsigned c = live ? (unsigned)c0 : 0u;
-            const unsigned J = c / (unsigned)A.nvec, v = c - J * (unsigned)A.nvec;
-            const unsigned Ja = J / (unsigned)A.NB, Jb = J - Ja * (unsigned)A.NB;
+            unsigned v, Ja, Jb;
+            if constexpr (ML > 0) {
+                v = 0;
+                Ja = c / (unsigned)A.nbp;
+                Jb = c - Ja * (unsigned)A.nbp;
+                if (Jb >= (unsigned)A.NB) {  // empty column of the padded layout
+                    live = false;
+                    Ja = Jb = 0;
+                }
+            } else {
+                const unsigned J = c / (unsigned)A.nvec;
+                v = c - J * (unsigned)A.nvec;
+                Ja = J / (unsigned)A.NB;
+                Jb = J - Ja * (unsigned)A.NB;
+            }
             // alpha rows string-major; beta entries slot-major (entry t of every beta
             // string contiguous, so the lanes' loads of one slot are one run)
             const int ra = (int)(Ja * (unsigned)A.nea), rb = A.offb + (int)Jb - A.nea * A.NB;
@@ -253,32 +278,57 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
     } else {
         const int lr = lane & 15, lk = lane >> 4;
         typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type acc_t;
-        auto aload = [&](int m, int q) -> T {
-            if constexpr (VM == VM_FULL) return vs[(16 * m + lr) * LS + 4 * q + lk];
-            else return V[(16 * m + lr) * KP + 4 * q + lk];
+        // pair of A row 16 m + lr: the row itself, or (live rows) the pair of live row
+        // 16 m + lr of the tile's alpha string, loaded one tile ahead with x
+        int prow[ML ? ML : 1], pnext[ML ? ML : 1];
+        auto rows_of = [&](int tile, int *r) {
+            if constexpr (ML > 0) {
+                const int Ka = min(tile, ntiles - 1) * 64 / A.nbp;
+#pragma unroll
+                for (int m = 0; m < ML; ++m) r[m] = A.live[Ka * RS + 16 * m + lr];
+            }
         };
-        // X of column c (determinant c / nvec, vector c % nvec), loaded one tile ahead
+        auto arow = [&](int m) -> int {
+            if constexpr (ML > 0) return prow[m];
+            else return 16 * m + lr;
+        };
+        auto aload = [&](int m, int q) -> T {
+            if constexpr (VM == VM_FULL) return vs[arow(m) * LS + 4 * q + lk];
+            else return V[arow(m) * KP + 4 * q + lk];
+        };
+        // X of column c (determinant c / nvec, vector c % nvec; live rows: determinant
+        // (c / nbp, c % nbp), empty columns clamped), loaded one tile ahead
         auto xat = [&](int c) -> T {
-            const int cc = min(c, A.ncol - 1), J = cc / A.nvec, v = cc - J * A.nvec;
+            const int cc = min(c, A.ncol - 1);
+            if constexpr (ML > 0) {
+                const int Ja = cc / A.nbp, Jb = min(cc - Ja * A.nbp, A.NB - 1);
+                return X[Ja * A.NB + Jb];
+            }
+            const int J = cc / A.nvec, v = cc - J * A.nvec;
             return A.colmajor ? X[v * A.nd + J] : X[J * A.vstride + A.v0 + v];
         };
         T x = xat((int)blockIdx.x * 64 + 16 * wave + lr);
+        rows_of((int)blockIdx.x, pnext);
+        const int nlim = ML ? A.nl : A.npair;  // rows that are stored
         // Two-buffer plans with the folded epilogue store a tile's E values during the
         // next tile's k loop, one per k step after its MFMAs (pv, pw, pok: the values,
         // their column and whether it exists), so no run of stores stands between the
         // MFMAs of consecutive tiles; the last tile's values are stored after the loop.
         // Rows past the last pair are not stored (no pass reads them).
         constexpr bool PIPE = FOLD && !SB;
-        T pv[MTL][4];
+        T pv[MR][4];
         T *pw = W;
         bool pok = false;
         auto row_of = [&](int m, int r) { return (sizeof(T) == 8) ? 16 * m + lk + 4 * r : 16 * m + 4 * lk + r; };
         auto product = [&](int tile, double *buf) {
             const int c = tile * 64 + 16 * wave + lr;
             double *bb = buf + 16 * wave + lr;
-            acc_t acc[MT];
+            if constexpr (ML > 0)
 #pragma unroll
-            for (int m = 0; m < MT; ++m) acc[m] = acc_t{};
+                for (int m = 0; m < ML; ++m) prow[m] = pnext[m];
+            acc_t acc[MR];
+#pragma unroll
+            for (int m = 0; m < MR; ++m) acc[m] = acc_t{};
             if constexpr (VM == VM_GLOBAL) {
                 // V through the caches (nbasis 15 in FP64 only): one k step at a time
 #pragma unroll 1
@@ -286,15 +336,15 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
                     const T bq = (T)bb[(4 * q + lk) * ld];
                     bb[(4 * q + lk) * ld] = 0.0;
 #pragma unroll
-                    for (int m = 0; m < MTL; ++m) acc[m] = mfma16(aload(m, q), bq, acc[m]);
+                    for (int m = 0; m < MR; ++m) acc[m] = mfma16(aload(m, q), bq, acc[m]);
                 }
             } else {
-                T ac[MTL], an[MTL], bc, bn;
+                T ac[MR], an[MR], bc, bn;
                 bc = (T)bb[lk * ld];
                 bb[lk * ld] = 0.0;
                 if (FOLD && NQ == 1 && lk == 3) bc = x;
 #pragma unroll
-                for (int m = 0; m < MTL; ++m) ac[m] = aload(m, 0);
+                for (int m = 0; m < MR; ++m) ac[m] = aload(m, 0);
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) {
                     if (q + 1 < NQ) {
@@ -302,49 +352,52 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
                         bb[(4 * q + 4 + lk) * ld] = 0.0;
                         if (FOLD && q + 2 == NQ && lk == 3) bn = x;  // row VC - 1
 #pragma unroll
-                        for (int m = 0; m < MTL; ++m) an[m] = aload(m, q + 1);
+                        for (int m = 0; m < MR; ++m) an[m] = aload(m, q + 1);
                     }
                     // the loads of step q + 1 go out before the MFMAs of step q, and the
                     // loads of later steps after them (no instruction crosses either point;
                     // not in the one-buffer plans, whose register budget it would exceed)
                     if constexpr (!SB) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int m = 0; m < MTL; ++m) acc[m] = mfma16(ac[m], bc, acc[m]);
+                    for (int m = 0; m < MR; ++m) acc[m] = mfma16(ac[m], bc, acc[m]);
                     if constexpr (!SB) __builtin_amdgcn_sched_barrier(0);
                     if constexpr (PIPE)
-                        if (q < MTL * 4 && pok && row_of(q / 4, q % 4) < A.npair)
+                        if (q < MR * 4 && pok && row_of(q / 4, q % 4) < nlim)
                             pw[row_of(q / 4, q % 4) * 64] = pv[q / 4][q % 4];
                     asm volatile("" ::: "memory");
 #pragma unroll
-                    for (int m = 0; m < MTL; ++m) ac[m] = an[m];
+                    for (int m = 0; m < MR; ++m) ac[m] = an[m];
                     bc = bn;
                 }
             }
             if constexpr (PIPE) {
                 // the previous tile's values beyond the k steps, then this tile's in their place
 #pragma unroll
-                for (int j = NQ; j < MTL * 4; ++j)
-                    if (pok && row_of(j / 4, j % 4) < A.npair) pw[row_of(j / 4, j % 4) * 64] = pv[j / 4][j % 4];
+                for (int j = NQ; j < MR * 4; ++j)
+                    if (pok && row_of(j / 4, j % 4) < nlim) pw[row_of(j / 4, j % 4) * 64] = pv[j / 4][j % 4];
 #pragma unroll
-                for (int m = 0; m < MTL; ++m)
+                for (int m = 0; m < MR; ++m)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) pv[m][r] = acc[m][r];
-                pw = W + (i64)tile * KP * 64 + 16 * wave + lr;
+                pw = W + (i64)tile * RS * 64 + 16 * wave + lr;
                 pok = c < A.ncol;
             } else if (c < A.ncol) {
-                T *w = W + (i64)tile * KP * 64 + 16 * wave + lr;
+                T *w = W + (i64)tile * RS * 64 + 16 * wave + lr;
+                const int *lrow = ML ? A.live + (tile * 64 / A.nbp) * RS : nullptr;
 #pragma unroll
-                for (int m = 0; m < MTL; ++m)
+                for (int m = 0; m < MR; ++m)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int p = row_of(m, r);
+                        if (ML && p >= nlim) continue;
                         if constexpr (FOLD)
                             w[p * 64] = acc[m][r];
                         else
-                            w[p * 64] = (T)0.5 * acc[m][r] + ks[p] * x;
+                            w[p * 64] = (T)0.5 * acc[m][r] + ks[ML ? lrow[p] : p] * x;
                     }
             }
             x = xat(c + 64 * (int)gridDim.x);
+            rows_of(tile + (int)gridDim.x, pnext);
         };
         if constexpr (!SB) {
             __syncthreads();
@@ -356,10 +409,10 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
             if constexpr (PIPE)
                 if (pok)
 #pragma unroll
-                    for (int m = 0; m < MTL; ++m)
+                    for (int m = 0; m < MR; ++m)
 #pragma unroll
                         for (int r = 0; r < 4; ++r)
-                            if (row_of(m, r) < A.npair) pw[row_of(m, r) * 64] = pv[m][r];
+                            if (row_of(m, r) < nlim) pw[row_of(m, r) * 64] = pv[m][r];
         } else {
             __syncthreads();
             for (int step = 0; step < nmine; ++step) {
@@ -425,8 +478,11 @@ __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int n
 // alpha string Ia and 64 consecutive beta strings, so the table row of Ia is the same
 // for the whole wave and comes through scalar loads; each chunk of CH entries issues
 // its CH E loads before adding any. Sums in the order of k_mf_sigma (same bits).
+// E value of entry (Ka, r) at column Ka * nbp + Ib, row r of tiles of KP rows: the
+// pair layout (nbp = NB, r = the pair) or the live-row one (nbp = NB rounded up to 64,
+// r = the rank of the pair among the live rows of Ka, KP = the live tile rows).
 template <typename T>
-__global__ void __launch_bounds__(256) k_mf_alpha(int na, int NB, int nchunk, int nea, int KP,
+__global__ void __launch_bounds__(256) k_mf_alpha(int na, int NB, int nbp, int nchunk, int nea, int KP,
                                                   const int2 *__restrict__ ta, const T *__restrict__ W,
                                                   T *__restrict__ Y) {
     const int wv = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -445,7 +501,7 @@ __global__ void __launch_bounds__(256) k_mf_alpha(int na, int NB, int nchunk, in
         for (int u = 0; u < CH; ++u) e[u] = row[min(t0 + u, nea - 1)];
         T w[CH];
 #pragma unroll
-        for (int u = 0; u < CH; ++u) w[u] = W[wt_at((i64)e[u].x * NB + ib, e[u].y & 0xffff, KP)];
+        for (int u = 0; u < CH; ++u) w[u] = W[wt_at((i64)e[u].x * nbp + ib, e[u].y & 0xffff, KP)];
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             const T sg = (t0 + u >= nea) ? (T)0 : (e[u].y >> 16) ? (T)-1 : (T)1;
@@ -689,6 +745,43 @@ void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn) {
     merged.push_back(int2{0, dprod_vc(D->kpad / 16)});  // sentinel: empty slots of the fused pass add into the sink row
     D->tab.alloc(merged.size());
     PE_HIP(hipMemcpyAsync(D->tab.p, merged.data(), sizeof(int2) * merged.size(), hipMemcpyHostToDevice, s));
+    // Parity-sector layout (nup == ndn): the alpha scatter reads, at column (Ka, Ib), only
+    // pairs {k, l} with one orbital occupied in Ka and the other empty, or k = l occupied:
+    // exactly the pairs of Ka's own table row. Their E rows alone are formed and stored,
+    // in that row's (pair) order; used where the fewer row tiles outweigh padding the
+    // beta strings to 64 (so that a 64-column tile holds one alpha string).
+    std::vector<int> live;
+    std::vector<int2> tal;
+    if (D->nup == D->ndn) {
+        const int ntot = 2 * D->nea, ml = ntot <= 64 ? 2 : ntot <= 96 ? 3 : 4;
+        const i64 nbp = (D->nbs + 63) / 64 * 64;
+        PE_REQUIRE(D->npair < D->kpad, PYCI_EUNSUPPORTED, "no zero row in the pair dimension");
+        if (ntot <= 128 && ml < pair_tiles(D->kpad / 16) && nbp * ml < D->nbs * pair_tiles(D->kpad / 16)) {
+            D->ml = ml;
+            D->nbp = nbp;
+            const int rs = 16 * ml;
+            live.assign((size_t)D->na * rs, D->kpad - 1);  // pad rows: row kpad - 1 of V and k is zero
+            std::vector<int> rank((size_t)D->na * D->kpad, -1);
+            for (i64 a = 0; a < D->na; ++a)
+                for (int t = 0; t < D->nea; ++t) {
+                    const int p = ta[(size_t)(a * D->nea + t)].y & 0xffff;
+                    live[(size_t)(a * rs + t)] = p;
+                    rank[(size_t)(a * D->kpad + p)] = t;
+                }
+            tal.resize(ta.size());
+            for (i64 a = 0; a < D->na; ++a)
+                for (int t = 0; t < D->nea; ++t) {
+                    const int2 e = ta[(size_t)(a * D->nea + t)];
+                    const int r = rank[(size_t)e.x * D->kpad + (e.y & 0xffff)];
+                    PE_REQUIRE(r >= 0, PYCI_EUNSUPPORTED, "string table: pair not among the live rows");
+                    tal[(size_t)(a * D->nea + t)] = int2{e.x, r | (e.y & ~0xffff)};
+                }
+            D->live.alloc(live.size());
+            D->tab_al.alloc(tal.size());
+            PE_HIP(hipMemcpyAsync(D->live.p, live.data(), sizeof(int) * live.size(), hipMemcpyHostToDevice, s));
+            PE_HIP(hipMemcpyAsync(D->tab_al.p, tal.data(), sizeof(int2) * tal.size(), hipMemcpyHostToDevice, s));
+        }
+    }
     D->str_a.alloc(sa.size());
     D->str_b.alloc(sb.size());
     PE_HIP(hipMemcpyAsync(D->tab_a.p, ta.data(), sizeof(int2) * ta.size(), hipMemcpyHostToDevice, s));
@@ -755,10 +848,10 @@ static int device_attr(hipDeviceAttribute_t a) {
 }
 static size_t lds_per_block() { return (size_t)device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock); }
 
-template <int MT, typename T, bool SB, int VM, int PE>
+template <int MT, typename T, bool SB, int VM, int PE, int ML>
 static void launch_dprod_v(const MfView &M, const DprodArgs &A, const T *X, const T *V, const T *kv, T *W, size_t lds,
                            hipStream_t s) {
-    auto kern = k_mf_dprod<MT, T, SB, VM, PE>;
+    auto kern = k_mf_dprod<MT, T, SB, VM, PE, ML>;
     // dynamic LDS above 64 KB, set once (a function-local static is initialised
     // once even when two host threads launch the kernel for the first time together)
     static const bool attr = [&] {
@@ -803,8 +896,10 @@ struct MfCall {
     const long long *dyn;
 };
 
+// ml > 0: the live-row layout (one row-major vector, parity-sector scatter)
 template <int MT, typename T>
-static void launch_dprod(const MfView &M, const MfCall &C, const T *X, const T *V, const T *kv, T *W, hipStream_t s) {
+static void launch_dprod(const MfView &M, const MfCall &C, const T *X, const T *V, const T *kv, T *W, hipStream_t s,
+                         int ml) {
     typedef DprodPlan<MT, T> P;
     static const size_t cap = lds_per_block();
     const int ntot = M.nea + M.neb;
@@ -818,10 +913,28 @@ static void launch_dprod(const MfView &M, const MfCall &C, const T *X, const T *
     PE_REQUIRE(P::VM != VM_FULL || fixed_npair(MT) || M.npair < dprod_vc(MT), PYCI_EUNSUPPORTED,
                "pair count fills the pair dimension");
     DprodArgs A{(int)(C.nd * C.nvec), (int)M.nbs, C.nvec, M.npair, M.nea, ntot, C.vstride, C.v0, C.colmajor,
-                (int)C.nd, C.dyn, (int)(M.na * M.nea), (int)(M.na * M.nea + M.nbs * M.neb), P::ld};
-    if (ntot <= 64) launch_dprod_v<MT, T, P::SB, P::VM, 8>(M, A, X, V, kv, W, P::lds, s);
-    else if (ntot <= 96) launch_dprod_v<MT, T, P::SB, P::VM, 12>(M, A, X, V, kv, W, P::lds, s);
-    else launch_dprod_v<MT, T, P::SB, P::VM, 16>(M, A, X, V, kv, W, P::lds, s);
+                (int)C.nd, C.dyn, (int)(M.na * M.nea), (int)(M.na * M.nea + M.nbs * M.neb), P::ld,
+                (int)M.nbp, M.nea, M.live};
+    if (ml > 0) {
+        // the live rows of a string (nea of them) and the table entries per column
+        // (2 nea) fix ml and PE together: ml = 2, 3, 4 with PE = 8, 12, 16
+        PE_REQUIRE(C.nvec == 1 && C.vstride == 1 && C.v0 == 0 && !C.colmajor && !C.dyn && M.live &&
+                       M.na * M.nbp < ((i64)1 << 31) - ((i64)1 << 20) && ntot == 2 * M.nea &&
+                       ml == (ntot <= 64 ? 2 : ntot <= 96 ? 3 : 4),
+                   PYCI_EUNSUPPORTED, "live-row layout does not fit this call");
+        A.ncol = (int)(M.na * M.nbp);
+        if (ml == 2) {
+            if constexpr (2 < pair_tiles(MT)) return launch_dprod_v<MT, T, P::SB, P::VM, 8, 2>(M, A, X, V, kv, W, P::lds, s);
+        } else if (ml == 3) {
+            if constexpr (3 < pair_tiles(MT)) return launch_dprod_v<MT, T, P::SB, P::VM, 12, 3>(M, A, X, V, kv, W, P::lds, s);
+        } else if (ml == 4) {
+            if constexpr (4 < pair_tiles(MT)) return launch_dprod_v<MT, T, P::SB, P::VM, 16, 4>(M, A, X, V, kv, W, P::lds, s);
+        }
+        throw EngineError(PYCI_EUNSUPPORTED, "no live-row product for this pair dimension");
+    }
+    if (ntot <= 64) launch_dprod_v<MT, T, P::SB, P::VM, 8, 0>(M, A, X, V, kv, W, P::lds, s);
+    else if (ntot <= 96) launch_dprod_v<MT, T, P::SB, P::VM, 12, 0>(M, A, X, V, kv, W, P::lds, s);
+    else launch_dprod_v<MT, T, P::SB, P::VM, 16, 0>(M, A, X, V, kv, W, P::lds, s);
 }
 
 // ev (optional): 3 events recorded before, between and after the two passes.
@@ -829,15 +942,17 @@ template <int NV, typename T>
 static void launch_passes(const MfView &M, const MfCall &C, const T *X, T *W, T *Y, const T *V, const T *kv,
                           hipStream_t s, hipEvent_t *ev, int sector) {
     if (ev) PE_HIP(hipEventRecord(ev[0], s));
+    const bool one_vec = C.nvec == 1 && C.vstride == 1 && C.v0 == 0 && !C.colmajor && !C.dyn;
+    const int ml = (sector != 0 && one_vec) ? M.ml : 0;
     switch (M.kpad / 16) {
-        case 1: launch_dprod<1, T>(M, C, X, V, kv, W, s); break;
-        case 2: launch_dprod<2, T>(M, C, X, V, kv, W, s); break;
-        case 3: launch_dprod<3, T>(M, C, X, V, kv, W, s); break;
-        case 4: launch_dprod<4, T>(M, C, X, V, kv, W, s); break;
-        case 5: launch_dprod<5, T>(M, C, X, V, kv, W, s); break;
-        case 6: launch_dprod<6, T>(M, C, X, V, kv, W, s); break;
-        case 7: launch_dprod<7, T>(M, C, X, V, kv, W, s); break;
-        case 8: launch_dprod<8, T>(M, C, X, V, kv, W, s); break;
+        case 1: launch_dprod<1, T>(M, C, X, V, kv, W, s, ml); break;
+        case 2: launch_dprod<2, T>(M, C, X, V, kv, W, s, ml); break;
+        case 3: launch_dprod<3, T>(M, C, X, V, kv, W, s, ml); break;
+        case 4: launch_dprod<4, T>(M, C, X, V, kv, W, s, ml); break;
+        case 5: launch_dprod<5, T>(M, C, X, V, kv, W, s, ml); break;
+        case 6: launch_dprod<6, T>(M, C, X, V, kv, W, s, ml); break;
+        case 7: launch_dprod<7, T>(M, C, X, V, kv, W, s, ml); break;
+        case 8: launch_dprod<8, T>(M, C, X, V, kv, W, s, ml); break;
         default: throw EngineError(PYCI_EUNSUPPORTED, "pair dimension too large for the matrix-free operator");
     }
     if (ev) PE_HIP(hipEventRecord(ev[1], s));
@@ -846,11 +961,15 @@ static void launch_passes(const MfView &M, const MfCall &C, const T *X, T *W, T 
         // alpha terms (every block of 256 consecutive determinants shares few alpha
         // strings, so their table rows are common and the E reads contiguous), then
         // the parity symmetrisation
-        if (C.nvec == 1 && C.vstride == 1 && C.v0 == 0 && !C.colmajor && !C.dyn) {
+        if (one_vec) {
             const int nchunk = (int)((M.nbs + 63) / 64);
             const i64 waves = M.na * nchunk;
-            k_mf_alpha<T><<<(unsigned)((waves + 3) / 4), 256, 0, s>>>((int)M.na, (int)M.nbs, nchunk, M.nea, M.kpad,
-                                                                     M.tab_a, W, Y);
+            if (ml > 0)
+                k_mf_alpha<T><<<(unsigned)((waves + 3) / 4), 256, 0, s>>>((int)M.na, (int)M.nbs, (int)M.nbp, nchunk,
+                                                                         M.nea, 16 * ml, M.tab_al, W, Y);
+            else
+                k_mf_alpha<T><<<(unsigned)((waves + 3) / 4), 256, 0, s>>>((int)M.na, (int)M.nbs, (int)M.nbs, nchunk,
+                                                                         M.nea, M.kpad, M.tab_a, W, Y);
         } else {
             k_mf_sigma<NV, T, false><<<grid_for(nd, 256), 256, 0, s>>>(nd, M.nbs, M.nea, M.neb, M.kpad, M.tab_a,
                                                                        M.tab_b, C.nvec, C.vstride, C.v0, 1,
@@ -891,7 +1010,20 @@ static void launch_passes_nv(const MfView &M, const MfCall &C, const T *X, T *W,
 }
 
 static MfView view_of(const DirectOp *D) {
-    return MfView{D->tab.p, D->tab_a.p, D->tab_b.p, D->na, D->nbs, D->npair, D->kpad, D->nea, D->neb};
+    MfView M{D->tab.p, D->tab_a.p, D->tab_b.p, D->na, D->nbs, D->npair, D->kpad, D->nea, D->neb};
+    M.ml = D->ml;
+    M.nbp = D->nbp;
+    M.live = D->live.p;
+    M.tab_al = D->tab_al.p;
+    return M;
+}
+
+// Workspace values of one sigma of nc vectors: kpad rows per column, or for the
+// live-row layout (one vector in a parity sector) 16 ml rows per padded column.
+static i64 workspace_elems(const DirectOp *D, i64 nd, int nc, int sector) {
+    i64 e = ((nd * nc + 63) / 64 * 64) * (i64)D->kpad;
+    if (D->ml > 0 && sector != 0 && nc == 1) e = std::max<i64>(e, D->na * D->nbp * 16 * D->ml);
+    return e;
 }
 
 // The vectors go through the two passes in chunks whose E workspace stays within
@@ -908,7 +1040,7 @@ void direct_sigma(const pyci_op *op, int nvec, const T *X, T *Y, hipStream_t s, 
     const i64 nd = op->nrow;
     const i64 per_vec = ((nd + 63) / 64 * 64) * (i64)D->kpad * (i64)sizeof(T);
     const int chunk = (int)std::max<i64>(1, std::min<i64>(nvec, ((i64)16 << 30) / per_vec));
-    T *W = direct_workspace<T>(op, ((nd * chunk + 63) / 64 * 64) * (i64)D->kpad, ws);
+    T *W = direct_workspace<T>(op, workspace_elems(D, nd, chunk, sector), ws);
     const T *V = std::is_same<T, double>::value ? (const T *)D->V64.p : (const T *)D->V32.p;
     const T *kv = std::is_same<T, double>::value ? (const T *)D->k64.p : (const T *)D->k32.p;
     const MfView M = view_of(D);
@@ -981,9 +1113,13 @@ template void direct_sigma<float>(const pyci_op *, int, const float *, float *, 
                                   DevBuf<unsigned char> *);
 
 // Per-pass timing of nrep matrix-free sigmas on random X (see pyci_op_bench_sigma_parts).
+// One vector with nup == ndn is timed as the solver runs it, in a parity sector (the
+// sector's sigma of random X costs what it costs on a vector of the sector).
 // Algorithmic bytes and flops per launch, with N = nd * nvec columns and T the value size:
-//   gather + product: reads X (T N), writes E (T npair N); 2 npair^2 N flops
-//   scatter pass:     reads E (T npair N), writes Y (T N)
+//   gather + product: reads X (T N), writes E (T R N); 2 R npair N flops, R = npair, or
+//                     in a sector with the live-row layout nea (the rows stored)
+//   scatter pass:     reads E (T R N), writes Y (T N); in a sector it then reads and
+//                     writes Y once more (the parity symmetrisation)
 void direct_bench_parts(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *ms, double *bytes, double *flops) {
     DirectOp *D = op->direct.get();
     hipStream_t s = op->ctx->stream;
@@ -1005,9 +1141,10 @@ void direct_bench_parts(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *ms, do
     }
     std::vector<hipEvent_t> ev((size_t)(3 * nrep));
     for (auto &e : ev) PE_HIP(hipEventCreate(&e));
+    const int sector = (D->nup == D->ndn && nvec == 1) ? 1 : 0;
     auto run = [&](hipEvent_t *e) {
-        if (fp32) direct_sigma<float>(op, (int)nvec, x32.p, y32.p, s, e);
-        else direct_sigma<double>(op, (int)nvec, x.p, y.p, s, e);
+        if (fp32) direct_sigma<float>(op, (int)nvec, x32.p, y32.p, s, e, sector);
+        else direct_sigma<double>(op, (int)nvec, x.p, y.p, s, e, sector);
     };
     try {
         run(nullptr);  // warm-up (workspace allocation)
@@ -1021,11 +1158,12 @@ void direct_bench_parts(pyci_op *op, i64 nvec, i64 nrep, int fp32, float *ms, do
                 acc[p] += t;
             }
         const double T = fp32 ? 4.0 : 8.0, P = (double)D->npair, NN = (double)N;
+        const double R = (sector && D->ml) ? (double)D->nea : P;
         for (int p = 0; p < 2; ++p) ms[p] = (float)(acc[p] / (double)nrep);
-        bytes[0] = T * NN * (1.0 + P);
-        bytes[1] = T * NN * (P + 1.0);
-        flops[0] = 2.0 * P * P * NN + 2.0 * (double)(D->nea + D->neb) * NN;
-        flops[1] = 2.0 * (double)(D->nea + D->neb) * NN;
+        bytes[0] = T * NN * (1.0 + R);
+        bytes[1] = T * NN * (R + 1.0 + (sector ? 2.0 : 0.0));
+        flops[0] = 2.0 * R * P * NN + 2.0 * (double)(D->nea + D->neb) * NN;
+        flops[1] = 2.0 * (double)(sector ? D->nea : D->nea + D->neb) * NN;
     } catch (...) {
         for (auto &e : ev) (void)hipEventDestroy(e);
         throw;

@@ -195,6 +195,12 @@ struct DirectOp {
     DevBuf<int2> tab_a, tab_b;   // [string][entry] {rank of the excited string, pair | negative << 16}
     DevBuf<int2> tab;            // tab_a rows, tab_b entries slot-major (entry t of every beta string together), then the sentinel
     DevBuf<u64> str_a, str_b;    // strings in colex order
+    // parity-sector layout of E (nup == ndn, where it saves work): only the pairs the alpha
+    // scatter reads, per alpha string; ml = their row tiles (0: layout not used)
+    int ml = 0;
+    i64 nbp = 0;                 // beta strings rounded up to 64
+    DevBuf<int> live;            // [alpha string][16 ml] pair of each live row (pad: a zero row of V)
+    DevBuf<int2> tab_al;         // tab_a with {rank of the excited string, live row | negative << 16}
     DevBuf<double> one_mo, two_mo, V64, k64;
     DevBuf<float> V32, k32;
 };
@@ -424,6 +430,11 @@ struct MfView {
     const int2 *tab, *tab_a, *tab_b;
     i64 na, nbs;
     int npair, kpad, nea, neb;
+    // parity-sector live-row layout (ml 0: none)
+    int ml = 0;
+    i64 nbp = 0;
+    const int *live = nullptr;
+    const int2 *tab_al = nullptr;
 };
 template <typename T>
 void mf_sigma_batched(const MfView &M, i64 nd, int nvec_max, const T *X, T *Y, T *W, const T *V, const T *kv,
