@@ -161,7 +161,10 @@ int pyci_op_bench_sigma(pyci_op *op, int64_t nvec, int64_t nrep, int fp32, float
 /* Per-kernel view of the same measurement: *nparts kernels per sigma (1 for the
  * CSR; 2 for the matrix-free operator: fused gather + pair-integral product, scatter
  * pass), each with its average duration (HIP events between the kernels of each
- * of nrep sigmas), algorithmic bytes and flops per launch. Arrays hold >= 4. */
+ * of nrep sigmas), algorithmic bytes and flops per launch. Arrays hold >= 4. For the
+ * matrix-free operator with nup == ndn and nvec == 1 the timed sigma is the
+ * parity-sector one the eigensolver runs (product of the rows its scatter reads;
+ * scatter = alpha terms + symmetrisation), on random X. */
 int pyci_op_bench_sigma_parts(pyci_op *op, int64_t nvec, int64_t nrep, int fp32, int *nparts, float *avg_ms,
                               double *bytes, double *flops);
 /* solve(n, c0, ncv, maxiter, tol) (sparseop.cpp:114-146,161-173): lowest n

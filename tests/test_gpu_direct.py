@@ -65,6 +65,25 @@ def test_direct_solve_matches_oracle(oracle, precision, n, nb, nu, nd):
         assert np.linalg.norm(a(c) - e * c) < 1e-7
 
 
+# nup == ndn, one root: every sigma of the solve is a parity-sector one of one vector, which
+# forms only the pair rows the alpha scatter reads (2, 3 or 4 live row tiles). The plans:
+# (15, 2) FP64 V through the caches (the epilogue adds k per live row), FP32 one buffer with the
+# same epilogue; (13, 7) FP64 one buffer with 4 live row tiles; (10, 5) two buffers (FP32 and the
+# FP64 corrections), 2 row tiles, the beta strings padded from 252 to 256.
+@pytest.mark.parametrize("nb,nu,precision", [(15, 2, "f64"), (15, 2, "mixed"), (13, 7, "f64"), (10, 5, "mixed")])
+def test_direct_live_rows_solve(nb, nu, precision):
+    ham, wfn, _, _ = fullci(nb, nu, nu, seed=nb + 70)
+    op = pyci.sparse_op(ham, wfn)
+    assert op.storage == "direct"
+    es, cs = op.solve(n=1, tol=1e-9, precision=precision)
+    c = cs[0]
+    # the converged vector through the general product (all pair rows, alpha and beta terms)
+    assert np.linalg.norm(op(c) - es[0] * c) < 1e-6
+    if len(wfn) <= 20000:
+        ref, _ = pyci.sparse_op(ham, wfn, storage="csr").solve(n=1, tol=1e-10)
+        assert abs(es[0] - ref[0]) < 1e-9
+
+
 def test_direct_fallbacks_and_update(oracle):
     _, one, two = seeded_integrals(6, 6, 2)
     ham = pyci.hamiltonian(0.0, one, two)
