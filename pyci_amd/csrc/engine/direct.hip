@@ -137,10 +137,10 @@ struct DprodArgs {
     int offb;  // start of the beta entries in the merged table, NA * nea (entry t of beta string r at offb + t NB + r)
     int sent;  // index of the sentinel entry
     int ld;
-    // live-row layout (ML > 0): column c = Ja * nbp + Jb (nbp: NB rounded up to 64, so a
-    // tile holds one alpha string; columns with Jb >= NB are empty), output row r of a
-    // tile of alpha string Ka is pair live[Ka * 16 ML + r] (r < nl; pad rows name a zero
-    // row of V)
+    // live-row layout (ML > 0): column c = Ja * nbp + Jb * nvec + v (nbp: NB nvec rounded
+    // up to 64, so a tile holds one alpha string; columns with Jb >= NB are empty), output
+    // row r of a tile of alpha string Ka is pair live[Ka * 16 ML + r] (r < nl; pad rows
+    // name a zero row of V)
     int nbp, nl;
     const int *live;
 };
@@ -197,12 +197,13 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
             const unsigned c = live ? (unsigned)c0 : 0u;
             unsigned v, Ja, Jb;
             if constexpr (ML > 0) {
-                v = 0;
                 Ja = c / (unsigned)A.nbp;
-                Jb = c - Ja * (unsigned)A.nbp;
+                const unsigned r = c - Ja * (unsigned)A.nbp;
+                Jb = r / (unsigned)A.nvec;
+                v = r - Jb * (unsigned)A.nvec;
                 if (Jb >= (unsigned)A.NB) {  // empty column of the padded layout
                     live = false;
-                    Ja = Jb = 0;
+                    Ja = Jb = v = 0;
                 }
             } else {
                 const unsigned J = c / (unsigned)A.nvec;
@@ -297,12 +298,12 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
             else return V[arow(m) * KP + 4 * q + lk];
         };
         // X of column c (determinant c / nvec, vector c % nvec; live rows: determinant
-        // (c / nbp, c % nbp), empty columns clamped), loaded one tile ahead
+        // (c / nbp, (c % nbp) / nvec), empty columns clamped), loaded one tile ahead
         auto xat = [&](int c) -> T {
             const int cc = min(c, A.ncol - 1);
             if constexpr (ML > 0) {
-                const int Ja = cc / A.nbp, Jb = min(cc - Ja * A.nbp, A.NB - 1);
-                return X[Ja * A.NB + Jb];
+                const int Ja = cc / A.nbp, r = cc - Ja * A.nbp, q = r / A.nvec, v = r - q * A.nvec;
+                return X[(Ja * A.NB + min(q, A.NB - 1)) * A.vstride + A.v0 + v];
             }
             const int J = cc / A.nvec, v = cc - J * A.nvec;
             return A.colmajor ? X[v * A.nd + J] : X[J * A.vstride + A.v0 + v];
@@ -425,17 +426,22 @@ __global__ void __launch_bounds__(768) k_mf_dprod(DprodArgs A, const int2 *__res
 }
 
 // Scatter pass: Y[I][v] = sum over the alpha and beta table entries of I of s E[pair][K v].
-// With alpha_only the beta terms are left out (see k_mf_symm).
+// With alpha_only the beta terms are left out (see k_mf_symm). The E value of alpha entry
+// (Ka, r) for vector v sits at column Ka * cs + Ib * nvec + v, row r of tiles of KP rows:
+// cs = NB nvec and r = the pair (pass cs = 0), or the live-row layout (alpha_only; cs =
+// NB nvec rounded up to 64, r = the live row, KP = the live tile rows).
 template <int NV, typename T, bool TILED>
 __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int neb, int KP,
                                                   const int2 *__restrict__ ta, const int2 *__restrict__ tb, int nvec,
                                                   int vstride, int v0, int alpha_only, int colmajor,
-                                                  const long long *dyn, const T *__restrict__ W, T *__restrict__ Y) {
+                                                  const long long *dyn, const T *__restrict__ W, T *__restrict__ Y,
+                                                  i64 cs) {
     if (dyn) {
         nvec = (int)dyn[0];
         Y += dyn[2];
         if (nvec == 0) return;
     }
+    if (cs == 0) cs = NB * nvec;
     i64 Ia, Ib;
     if (!det_of<TILED>(nd, nd / NB, NB, Ia, Ib)) return;
     const i64 I = Ia * NB + Ib;
@@ -455,7 +461,7 @@ __global__ void __launch_bounds__(256) k_mf_sigma(i64 nd, i64 NB, int nea, int n
             T w[CH][NV];
 #pragma unroll
             for (int u = 0; u < CH; ++u) {
-                const i64 c = (alpha ? (i64)e[u].x * NB + Ib : Ia * NB + e[u].x) * nvec;
+                const i64 c = alpha ? (i64)e[u].x * cs + Ib * nvec : (Ia * NB + e[u].x) * nvec;
 #pragma unroll
                 for (int v = 0; v < NV; ++v) w[u][v] = W[wt_at(c + min(v, nvec - 1), e[u].y & 0xffff, KP)];
             }
@@ -896,10 +902,11 @@ struct MfCall {
     const long long *dyn;
 };
 
-// ml > 0: the live-row layout (one row-major vector, parity-sector scatter)
+// ml > 0: the live-row layout (row-major vectors, parity-sector scatter) with nbp
+// columns per alpha string
 template <int MT, typename T>
 static void launch_dprod(const MfView &M, const MfCall &C, const T *X, const T *V, const T *kv, T *W, hipStream_t s,
-                         int ml) {
+                         int ml, i64 nbp) {
     typedef DprodPlan<MT, T> P;
     static const size_t cap = lds_per_block();
     const int ntot = M.nea + M.neb;
@@ -914,15 +921,15 @@ static void launch_dprod(const MfView &M, const MfCall &C, const T *X, const T *
                "pair count fills the pair dimension");
     DprodArgs A{(int)(C.nd * C.nvec), (int)M.nbs, C.nvec, M.npair, M.nea, ntot, C.vstride, C.v0, C.colmajor,
                 (int)C.nd, C.dyn, (int)(M.na * M.nea), (int)(M.na * M.nea + M.nbs * M.neb), P::ld,
-                (int)M.nbp, M.nea, M.live};
+                (int)nbp, M.nea, M.live};
     if (ml > 0) {
         // the live rows of a string (nea of them) and the table entries per column
         // (2 nea) fix ml and PE together: ml = 2, 3, 4 with PE = 8, 12, 16
-        PE_REQUIRE(C.nvec == 1 && C.vstride == 1 && C.v0 == 0 && !C.colmajor && !C.dyn && M.live &&
-                       M.na * M.nbp < ((i64)1 << 31) - ((i64)1 << 20) && ntot == 2 * M.nea &&
+        PE_REQUIRE(!C.colmajor && !C.dyn && M.live && nbp % 64 == 0 && nbp >= M.nbs * C.nvec &&
+                       M.na * nbp < ((i64)1 << 31) - ((i64)1 << 20) && ntot == 2 * M.nea &&
                        ml == (ntot <= 64 ? 2 : ntot <= 96 ? 3 : 4),
                    PYCI_EUNSUPPORTED, "live-row layout does not fit this call");
-        A.ncol = (int)(M.na * M.nbp);
+        A.ncol = (int)(M.na * nbp);
         if (ml == 2) {
             if constexpr (2 < pair_tiles(MT)) return launch_dprod_v<MT, T, P::SB, P::VM, 8, 2>(M, A, X, V, kv, W, P::lds, s);
         } else if (ml == 3) {
@@ -943,16 +950,19 @@ static void launch_passes(const MfView &M, const MfCall &C, const T *X, T *W, T 
                           hipStream_t s, hipEvent_t *ev, int sector) {
     if (ev) PE_HIP(hipEventRecord(ev[0], s));
     const bool one_vec = C.nvec == 1 && C.vstride == 1 && C.v0 == 0 && !C.colmajor && !C.dyn;
-    const int ml = (sector != 0 && one_vec) ? M.ml : 0;
+    // the live-row layout for row-major vectors in a sector: NB nvec columns per alpha
+    // string, padded to a multiple of 64
+    const int ml = (sector != 0 && !C.colmajor && !C.dyn) ? M.ml : 0;
+    const i64 nbp = ml ? (M.nbs * C.nvec + 63) / 64 * 64 : 0;
     switch (M.kpad / 16) {
-        case 1: launch_dprod<1, T>(M, C, X, V, kv, W, s, ml); break;
-        case 2: launch_dprod<2, T>(M, C, X, V, kv, W, s, ml); break;
-        case 3: launch_dprod<3, T>(M, C, X, V, kv, W, s, ml); break;
-        case 4: launch_dprod<4, T>(M, C, X, V, kv, W, s, ml); break;
-        case 5: launch_dprod<5, T>(M, C, X, V, kv, W, s, ml); break;
-        case 6: launch_dprod<6, T>(M, C, X, V, kv, W, s, ml); break;
-        case 7: launch_dprod<7, T>(M, C, X, V, kv, W, s, ml); break;
-        case 8: launch_dprod<8, T>(M, C, X, V, kv, W, s, ml); break;
+        case 1: launch_dprod<1, T>(M, C, X, V, kv, W, s, ml, nbp); break;
+        case 2: launch_dprod<2, T>(M, C, X, V, kv, W, s, ml, nbp); break;
+        case 3: launch_dprod<3, T>(M, C, X, V, kv, W, s, ml, nbp); break;
+        case 4: launch_dprod<4, T>(M, C, X, V, kv, W, s, ml, nbp); break;
+        case 5: launch_dprod<5, T>(M, C, X, V, kv, W, s, ml, nbp); break;
+        case 6: launch_dprod<6, T>(M, C, X, V, kv, W, s, ml, nbp); break;
+        case 7: launch_dprod<7, T>(M, C, X, V, kv, W, s, ml, nbp); break;
+        case 8: launch_dprod<8, T>(M, C, X, V, kv, W, s, ml, nbp); break;
         default: throw EngineError(PYCI_EUNSUPPORTED, "pair dimension too large for the matrix-free operator");
     }
     if (ev) PE_HIP(hipEventRecord(ev[1], s));
@@ -965,15 +975,15 @@ static void launch_passes(const MfView &M, const MfCall &C, const T *X, T *W, T 
             const int nchunk = (int)((M.nbs + 63) / 64);
             const i64 waves = M.na * nchunk;
             if (ml > 0)
-                k_mf_alpha<T><<<(unsigned)((waves + 3) / 4), 256, 0, s>>>((int)M.na, (int)M.nbs, (int)M.nbp, nchunk,
+                k_mf_alpha<T><<<(unsigned)((waves + 3) / 4), 256, 0, s>>>((int)M.na, (int)M.nbs, (int)nbp, nchunk,
                                                                          M.nea, 16 * ml, M.tab_al, W, Y);
             else
                 k_mf_alpha<T><<<(unsigned)((waves + 3) / 4), 256, 0, s>>>((int)M.na, (int)M.nbs, (int)M.nbs, nchunk,
                                                                          M.nea, M.kpad, M.tab_a, W, Y);
         } else {
-            k_mf_sigma<NV, T, false><<<grid_for(nd, 256), 256, 0, s>>>(nd, M.nbs, M.nea, M.neb, M.kpad, M.tab_a,
-                                                                       M.tab_b, C.nvec, C.vstride, C.v0, 1,
-                                                                       C.colmajor, C.dyn, W, Y);
+            k_mf_sigma<NV, T, false><<<grid_for(nd, 256), 256, 0, s>>>(
+                nd, M.nbs, M.nea, M.neb, ml ? 16 * ml : M.kpad, ml ? M.tab_al : M.tab_a, M.tab_b, C.nvec, C.vstride,
+                C.v0, 1, C.colmajor, C.dyn, W, Y, nbp);
         }
         PE_LAUNCH_CHECK();
         if (C.nvec == 1 && C.vstride == 1 && C.v0 == 0 && !C.colmajor && !C.dyn) {
@@ -991,10 +1001,10 @@ static void launch_passes(const MfView &M, const MfCall &C, const T *X, T *W, T 
     const unsigned g = tiled ? tiled_grid(M.na, M.nbs) : grid_for(nd, 256);
     if (tiled)
         k_mf_sigma<NV, T, true><<<g, 256, 0, s>>>(nd, M.nbs, M.nea, M.neb, M.kpad, M.tab_a, M.tab_b, C.nvec,
-                                                  C.vstride, C.v0, 0, C.colmajor, C.dyn, W, Y);
+                                                  C.vstride, C.v0, 0, C.colmajor, C.dyn, W, Y, 0);
     else
         k_mf_sigma<NV, T, false><<<g, 256, 0, s>>>(nd, M.nbs, M.nea, M.neb, M.kpad, M.tab_a, M.tab_b, C.nvec,
-                                                   C.vstride, C.v0, 0, C.colmajor, C.dyn, W, Y);
+                                                   C.vstride, C.v0, 0, C.colmajor, C.dyn, W, Y, 0);
     PE_LAUNCH_CHECK();
     if (ev) PE_HIP(hipEventRecord(ev[2], s));
 }
@@ -1019,10 +1029,11 @@ static MfView view_of(const DirectOp *D) {
 }
 
 // Workspace values of one sigma of nc vectors: kpad rows per column, or for the
-// live-row layout (one vector in a parity sector) 16 ml rows per padded column.
+// live-row layout (row-major vectors in a parity sector) 16 ml rows per column of the
+// padded layout (NB nc columns per alpha string rounded up to 64).
 static i64 workspace_elems(const DirectOp *D, i64 nd, int nc, int sector) {
     i64 e = ((nd * nc + 63) / 64 * 64) * (i64)D->kpad;
-    if (D->ml > 0 && sector != 0 && nc == 1) e = std::max<i64>(e, D->na * D->nbp * 16 * D->ml);
+    if (D->ml > 0 && sector != 0) e = std::max<i64>(e, D->na * ((D->nbs * nc + 63) / 64 * 64) * 16 * D->ml);
     return e;
 }
 

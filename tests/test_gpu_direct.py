@@ -65,6 +65,29 @@ def test_direct_solve_matches_oracle(oracle, precision, n, nb, nu, nd):
         assert np.linalg.norm(a(c) - e * c) < 1e-7
 
 
+# The largest LDS plans (pair dimensions 96..128: one buffer, V through the caches) against
+# the oracle itself: its CSR (the reference's build restated on the CPU) and its product.
+@pytest.mark.parametrize("nb,nu,nd", [(13, 2, 2), (14, 2, 1), (15, 2, 2)])
+def test_direct_large_pair_dimension_matches_oracle(oracle, nb, nu, nd):
+    ham, wfn, one, two = fullci(nb, nu, nd, seed=nb + 90)
+    dets = oracle.all_dets("fullci", nb, nu, nd)
+    np.testing.assert_array_equal(wfn.to_det_array().reshape(dets.shape), dets)
+    ref = oracle.csr_build("fullci", nb, nu, nd, dets, one, two)
+    a = pyci.sparse_op(ham, wfn, storage="csr")
+    np.testing.assert_array_equal(a.indptr(), ref.indptr)
+    np.testing.assert_array_equal(a.indices(), ref.indices)
+    np.testing.assert_array_equal(a.data(), ref.data)
+    b = pyci.sparse_op(ham, wfn)
+    assert b.storage == "direct"
+    rng = np.random.default_rng(nb + 90)
+    for _ in range(2):
+        x = rng.normal(size=len(wfn))
+        y, yr = b(x), oracle.matvec(ref, x)
+        # the same Hamiltonian summed in another order (1e-12 relative)
+        assert np.max(np.abs(y - yr)) <= 1e-12 * np.max(np.abs(yr))
+    np.testing.assert_array_equal(b.diagonal(), oracle.diagonal("fullci", nb, nu, nd, dets, one, two))
+
+
 # nup == ndn, one root: every sigma of the solve is a parity-sector one of one vector, which
 # forms only the pair rows the alpha scatter reads (2, 3 or 4 live row tiles). The plans:
 # (15, 2) FP64 V through the caches (the epilogue adds k per live row), FP32 one buffer with the
