@@ -204,16 +204,49 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
         // over the wave and the virtual pairs from a per-wave LDS table built once per
         // row (no per-candidate index decode). The order of the candidates is free:
         // the row is sorted by column afterwards.
-        for (int e0 = 0; e0 < nsu; e0 += 64) {
-            const int e = e0 + lane, ec = e < nsu ? e : 0, i = ec / nvu, a = ec - i * nvu;
+        // With w.szh > 0 every determinant of the set has the same number of occupied
+        // orbitals below szh, so only excitations that keep that number can find a
+        // column: singles within each half, doubles whose virtual pair has as many
+        // orbitals below szh as the occupied pair (0, 1 or 2). The orbital lists are
+        // sorted, so the lower orbitals come first (nol occupied, nvl virtual); the
+        // virtual pair table holds the three classes in turn (both low, one low, none),
+        // each pair as (a < l) like the unfiltered table, so every visited candidate
+        // gets the same orbitals, phase and value. The count and fill passes enumerate
+        // the same chunks either way.
+        const u64 lowm = w.szh > 0 ? full_mask(w.szh) : 0ULL;
+        const int nol = w.szh > 0 ? __popcll(up & lowm) : nou, nvl = w.szh > 0 ? __popcll(~up & m & lowm) : nvu;
+        const int noh = nou - nol, nvh = nvu - nvl;
+        const int sl = nol * nvl, ns = sl + noh * nvh;
+        for (int e0 = 0; e0 < ns; e0 += 64) {
+            const int e = e0 + lane, ec = e < ns ? e : 0;
+            int i, a;
+            if (ec < sl) {
+                i = ec / nvl;
+                a = ec - i * nvl;
+            } else {
+                const int t = ec - sl;
+                i = nol + t / nvh;
+                a = nvl + (t - (i - nol) * nvh);
+            }
             const int ii = __shfl(packed, i) & 255, jj = (__shfl(packed, a) >> 8) & 255;
-            visit(e < nsu, 0, ii, jj, 0, 0, 0, 0, 0);
+            visit(e < ns, 0, ii, jj, 0, 0, 0, 0, 0);
         }
+        const int nLL = nvl * (nvl - 1) / 2, nLH = nvl * nvh, nHH = nvh * (nvh - 1) / 2;
         unsigned short *vp = genci_vp + (threadIdx.x >> 6) * GENCI_MAXPV;
         for (int t0 = 0; t0 < npv_u; t0 += 64) {
-            const int t = t0 + lane;
+            const int t = t0 + lane, tt = t < npv_u ? t : 0;
             int a, l;
-            tri_decode(t < npv_u ? t : 0, a, l);
+            if (tt < nLL) {
+                tri_decode(tt, a, l);
+            } else if (tt < nLL + nLH) {
+                const int u = tt - nLL;
+                a = u / nvh;
+                l = nvl + (u - a * nvh);
+            } else {
+                tri_decode(tt - nLL - nLH, a, l);
+                a += nvl;
+                l += nvl;
+            }
             const int va = (__shfl(packed, a) >> 8) & 255, vl = (__shfl(packed, l) >> 8) & 255;
             if (t < npv_u) vp[t] = (unsigned short)(va | (vl << 8));
         }
@@ -223,10 +256,13 @@ __global__ void __launch_bounds__(256) k_rows(WfnView w, HamView H, i64 row0, i6
             int i, k;
             tri_decode(po, i, k);
             const int ii = __shfl(packed, i) & 255, kk = __shfl(packed, k) & 255;
-            for (int t0 = 0; t0 < npv_u; t0 += 64) {
+            const int nlo = (i < nol) + (k < nol);  // occupied orbitals of the pair below szh
+            const int s0 = nlo == 2 ? 0 : nlo == 1 ? nLL : nLL + nLH;
+            const int sn = w.szh == 0 ? npv_u : nlo == 2 ? nLL : nlo == 1 ? nLH : nHH;
+            for (int t0 = 0; t0 < sn; t0 += 64) {
                 const int t = t0 + lane;
-                const unsigned pv = vp[t < npv_u ? t : 0];
-                visit(t < npv_u, 2, ii, (int)(pv & 255), kk, (int)(pv >> 8), 0, 0, 0);
+                const unsigned pv = vp[(w.szh == 0 ? 0 : s0) + (t < sn ? t : 0)];
+                visit(t < sn, 2, ii, (int)(pv & 255), kk, (int)(pv >> 8), 0, 0, 0);
             }
         }
     } else if constexpr (SORTED) {
@@ -693,6 +729,42 @@ __global__ void k_sorted_meta(i64 row0, i64 nr, i64 rowlen, i64 nsd, SortedRows 
     }
 }
 
+// lo[0] = min, lo[1] = max over the determinants of the occupied orbitals below h
+__global__ void k_low_range(const u64 *dets, i64 nd, int nkey, int h, int *lo) {
+    int mn = 64, mx = 0;
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < nd; i += (i64)gridDim.x * blockDim.x) {
+        const int c = __popcll(dets[i * nkey] & full_mask(h));
+        mn = min(mn, c);
+        mx = max(mx, c);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = min(mn, __shfl_xor(mn, o));
+        mx = max(mx, __shfl_xor(mx, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(lo, mn);
+        atomicMax(lo + 1, mx);
+    }
+}
+
+// GenCI: nbasis / 2 if every determinant of the set has the same number of occupied
+// spin orbitals in the lower half (a set of one Sz in the alpha-then-beta ordering),
+// else 0. An excitation that changes that number then cannot find a column.
+static int genci_split(const pyci_wfn *wfn, hipStream_t s) {
+    const int n = (int)wfn->nbasis;
+    if (wfn->kind != PYCI_WFN_GENCI || n < 2 || n % 2 != 0 || wfn->ndet < 1) return 0;
+    DevBuf<int> lo(2);
+    const int init[2] = {64, 0};
+    PE_HIP(hipMemcpyAsync(lo.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    k_low_range<<<(unsigned)std::min<i64>(1024, (wfn->ndet + 255) / 256), 256, 0, s>>>(wfn->dets.p, wfn->ndet,
+                                                                                     (int)wfn->nkey, n / 2, lo.p);
+    PE_LAUNCH_CHECK();
+    int h[2] = {0, 1};
+    PE_HIP(hipMemcpyAsync(h, lo.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    PE_HIP(hipStreamSynchronize(s));
+    return h[0] == h[1] ? n / 2 : 0;
+}
+
 template <int KIND>
 static void launch_rows(bool fill, const WfnView &w, const HamView &H, i64 row0, i64 row1, i64 ncol, i64 *cf,
                         i64 *cl, int *mx, const i64 *rp, int32_t *cols, double *vals, double *diag, hipStream_t s,
@@ -802,6 +874,7 @@ void build_rows(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn, i64 row0,
     if (build_rows_sorted(op, wfn, w, H, row0, row1, ncol, rowptr_seg, cols, vals, lowcnt_seg, diag_seg, nnz_seg,
                           low_total))
         return;
+    w.szh = genci_split(wfn, s);
     rowptr_seg.alloc((size_t)(nr + 1));
     lowcnt_seg.alloc((size_t)(nr + 1));
     diag_seg.alloc((size_t)(nr > 0 ? nr : 1));

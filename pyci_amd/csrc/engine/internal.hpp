@@ -265,6 +265,10 @@ struct WfnView {
     const u64 *htags;
     const unsigned *hfilt;  // or null
     u64 hmask, fmask;
+    // GenCI row builds: every determinant of the set has the same number of occupied
+    // orbitals below szh (0: not known to), so an excitation that changes that number
+    // cannot find a column and is not enumerated
+    int szh = 0;
 };
 
 struct HamView {

@@ -72,6 +72,34 @@ def test_subset_wfn_hash_index(oracle, symmetric):
     check_csr(op, oracle.csr_build("fullci", 7, 3, 2, dets, one, two, nrow=nrow, symmetric=symmetric))
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_genci_fixed_lower_half_count(oracle, mixed, symmetric):
+    """GenCI sets whose determinants all hold the same number of occupied spin orbitals
+    in the lower half (one Sz in the alpha-then-beta ordering) do not enumerate the
+    excitations that change that number, which cannot find a column; one determinant of
+    another count (mixed) turns that off. General spin-orbital integrals (no spin blocks),
+    so the skipped excitations would have nonzero elements: both CSRs against the oracle,
+    bit for bit, with row ranges (nrow) on the symmetric one."""
+    nb, ne = 12, 4
+    _, one, two = seeded_integrals(nb, 6, 33)
+    alld = oracle.all_dets("genci", nb, ne, 0).reshape(-1)
+    low = np.array([bin(int(d) & 63).count("1") for d in alld])
+    sel = alld[low == 2]
+    if mixed:
+        sel = np.concatenate([sel[:100], alld[low == 3][:1], sel[100:]])
+    dets = sel.reshape(-1, 1)
+    w = make_wfn("genci", nb, ne, 0, dets)
+    ham = pyci.hamiltonian(0.0, one, two)
+    if symmetric:
+        nrow = len(sel) - 7
+        op = pyci.sparse_op(ham, w, nrow, symmetric=True)
+        check_csr(op, oracle.csr_build("genci", nb, ne, 0, dets, one, two, nrow=nrow, symmetric=True))
+    else:
+        op = pyci.sparse_op(ham, w, symmetric=False)
+        check_csr(op, oracle.csr_build("genci", nb, ne, 0, dets, one, two, symmetric=False))
+
+
 def test_rectangular_matvec_equals_get_element_sums(oracle):
     """test_routines.py:57-80 on be_ccpvdz FullCI HF+singles+doubles."""
     ham = pyci.secondquant_op(datafile("be_ccpvdz.fcidump"))
