@@ -666,30 +666,35 @@ i64 rank_host(u64 det, const std::vector<i64> &binom) {
 }
 
 std::vector<int2> string_table(int n, const std::vector<u64> &S, int ne, const std::vector<i64> &binom) {
+    PE_REQUIRE(n >= 1 && n <= 20, PYCI_EUNSUPPORTED, "string tables are built for nbasis <= 20");
+    (void)binom;
+    // S holds every string of its popcount in colex (= increasing) order, so the rank
+    // of a string is its position in S: a direct table over all 2^n masks
+    std::vector<int> rk((size_t)1 << n, -1);
+    for (size_t r = 0; r < S.size(); ++r) rk[(size_t)S[r]] = (int)r;
     std::vector<int2> t(S.size() * (size_t)ne);
-    std::vector<std::pair<int, int2>> row;
+    // Pairs p = i (i + 1) / 2 + j (i >= j) are visited in increasing order, so each row
+    // comes out sorted by pair: (i, i) for i occupied, (i, j) for exactly one of i, j
+    // occupied (k the occupied one, l the other).
     for (size_t r = 0; r < S.size(); ++r) {
         const u64 s = S[r];
-        row.clear();
-        for (int k = 0; k < n; ++k) {
-            if (!((s >> k) & 1ULL)) continue;
-            for (int l = 0; l < n; ++l) {
-                if (l != k && ((s >> l) & 1ULL)) continue;
+        int e = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j <= i; ++j) {
+                const int oi = (int)((s >> i) & 1ULL), oj = (int)((s >> j) & 1ULL);
+                if (i == j ? !oi : oi == oj) continue;
                 u64 K = s;
                 int neg = 0;
-                if (l != k) {
-                    K = s ^ (1ULL << k) ^ (1ULL << l);
-                    const int lo = std::min(k, l), hi = std::max(k, l);
-                    const u64 between = ((hi >= 64) ? ~0ULL : ((1ULL << hi) - 1ULL)) & ~((1ULL << (lo + 1)) - 1ULL);
+                if (i != j) {
+                    K = s ^ (1ULL << i) ^ (1ULL << j);
+                    const u64 between = ((1ULL << i) - 1ULL) & ~((1ULL << (j + 1)) - 1ULL);
                     neg = __builtin_popcountll(s & between) & 1;
                 }
-                const int i = std::max(k, l), j = std::min(k, l);
-                const int p = i * (i + 1) / 2 + j;
-                row.push_back({p, int2{(int)rank_host(K, binom), p | (neg << 16)}});
+                const int p = i * (i + 1) / 2 + j, rank = rk[(size_t)K];
+                PE_REQUIRE(e < ne && rank >= 0, PYCI_EINVAL, "string table: the strings are not a full colex list");
+                t[r * ne + e++] = int2{rank, p | (neg << 16)};
             }
-        }
-        std::sort(row.begin(), row.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
-        for (int e = 0; e < ne; ++e) t[r * ne + e] = row[(size_t)e].second;
+        PE_REQUIRE(e == ne, PYCI_EINVAL, "string table: unexpected number of entries");
     }
     return t;
 }
@@ -742,7 +747,8 @@ void direct_setup(pyci_op *op, const pyci_ham *ham, const pyci_wfn *wfn) {
     PE_REQUIRE(D->na * D->nbs == wfn->ndet, PYCI_EINVAL, "wave function is not the full FullCI space");
     D->nea = D->nup * (n - D->nup + 1);
     D->neb = D->ndn * (n - D->ndn + 1);
-    const std::vector<int2> ta = string_table(n, sa, D->nea, binom), tb = string_table(n, sb, D->neb, binom);
+    const std::vector<int2> ta = string_table(n, sa, D->nea, binom);
+    const std::vector<int2> tb = (D->nup == D->ndn) ? ta : string_table(n, sb, D->neb, binom);  // same strings
     D->tab_a.alloc(ta.size());
     D->tab_b.alloc(tb.size());
     std::vector<int2> merged(ta);
